@@ -1,0 +1,389 @@
+// Batch engine: builds the canonical device state of a batch of simulations
+// from tumour states, and runs it with the persistent kernel, the per-step
+// kernels, or the per-step kernels replayed from a captured HIP graph.
+#include "engine.h"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+namespace emt6ro {
+
+struct ImgDesc {
+  int64_t sub_off, st_off, slot_off;
+  int32_t words;  // slot words of buffer 0 (kSlotFields * cap)
+  int32_t pad;
+};
+
+namespace {
+
+// Copies each simulation's tumour image into its own state and header.
+__global__ void replicate_kernel(const SimDesc *desc, const int32_t *tumor_of,
+                                 const ImgDesc *img, const SimHdr *img_hdr,
+                                 const float *img_sub, const uint8_t *img_st,
+                                 const uint32_t *img_slots, float *sub, uint8_t *st,
+                                 uint32_t *slots, SimHdr *hdr, int nsims) {
+  const int sim = blockIdx.x;
+  if (sim >= nsims) return;
+  const SimDesc d = desc[sim];
+  const int t = tumor_of[sim];
+  const ImgDesc im = img[t];
+  const int hw = d.H * d.W;
+  for (int i = threadIdx.x; i < 3 * hw; i += blockDim.x) sub[d.sub_off + i] = img_sub[im.sub_off + i];
+  const int stw = (int)(st_bytes(d.H, d.W) / 4);
+  const uint32_t *s32 = reinterpret_cast<const uint32_t *>(img_st + im.st_off);
+  uint32_t *d32 = reinterpret_cast<uint32_t *>(st + d.st_off);
+  for (int i = threadIdx.x; i < stw; i += blockDim.x) d32[i] = s32[i];
+  for (int i = threadIdx.x; i < im.words; i += blockDim.x)
+    slots[d.slot_off + i] = img_slots[im.slot_off + i];
+  if (threadIdx.x == 0) hdr[sim] = img_hdr[t];
+}
+
+template <typename T>
+T *dalloc(size_t n) {
+  void *p = nullptr;
+  if (n == 0) n = 1;
+  EMT_HIP(hipMalloc(&p, n * sizeof(T)));
+  return static_cast<T *>(p);
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+}  // namespace
+
+Batch::Batch(const HostParams &p, const std::vector<const TumorState *> &tumors,
+             const BatchSpec &spec, int device, uint64_t seed)
+    : params_(p), dparams_(p.device()), device_(device),
+      seed_((uint32_t)(seed ^ (seed >> 32))), nsims_((int)spec.tumor_index.size()),
+      spec_(spec) {
+  EMT_ENFORCE(nsims_ > 0, -1, "A batch needs at least one simulation");
+  EMT_ENFORCE(!tumors.empty(), -1, "A batch needs at least one tumour");
+  EMT_ENFORCE(spec.prot_index.size() == (size_t)nsims_ && spec.horizon.size() == (size_t)nsims_ &&
+                  spec.sim_id.size() == (size_t)nsims_, -1, "Inconsistent batch arrays");
+  EMT_ENFORCE(spec.prot_res > 0, -1, "Protocol resolution must be positive");
+  EMT_ENFORCE(spec.n_prot > 0 && spec.n_slots > 0, -1, "Protocol table must be non-empty");
+  EMT_HIP(hipSetDevice(device_));
+  // Per-tumour images.
+  const int T = (int)tumors.size();
+  std::vector<ImgDesc> img(T);
+  int64_t isub = 0, ist = 0, islot = 0;
+  for (int t = 0; t < T; ++t) {
+    const TumorState &s = *tumors[t];
+    EMT_ENFORCE(s.H >= 3 && s.W >= 3 && s.H <= 255 && s.W <= 255, -1, "Bad tumour dims");
+    img[t] = ImgDesc{isub, ist, islot, kSlotFields * slot_cap(s.H, s.W), 0};
+    isub += 3 * (int64_t)s.sites();
+    ist += st_bytes(s.H, s.W);
+    islot += (int64_t)kSlotFields * slot_cap(s.H, s.W);
+  }
+  std::vector<float> h_sub(isub);
+  std::vector<uint8_t> h_st(ist, 0);
+  std::vector<uint32_t> h_slots(islot, 0);
+  img_hdr_.assign(T, SimHdr{});
+  for (int t = 0; t < T; ++t) {
+    const TumorState &s = *tumors[t];
+    const size_t hw = s.sites();
+    for (int k = 0; k < 3; ++k)
+      std::memcpy(&h_sub[img[t].sub_off + k * hw], s.f[k].data(), hw * sizeof(float));
+    std::memcpy(&h_st[img[t].st_off], s.occ.data(), hw);
+    const int cap = slot_cap(s.H, s.W);
+    const std::vector<int32_t> list = occupied_list_order(s);
+    EMT_ENFORCE((int)list.size() <= cap, -1, "Too many occupied sites");
+    uint32_t *S = &h_slots[img[t].slot_off];
+    for (size_t i = 0; i < list.size(); ++i) {
+      const int32_t x = list[i];
+      auto put = [&](int field, float v) { std::memcpy(&S[(size_t)field * cap + i], &v, 4); };
+      put(kTRep, s.f[TumorState::kTRep][x]);
+      put(kR, s.f[TumorState::kIrr][x]);
+      put(kDelay, s.f[TumorState::kDelay][x]);
+      put(kProlif, s.f[TumorState::kProlif][x]);
+      for (int k = 0; k < 5; ++k) put(kT0 + k, s.f[TumorState::kG1 + k][x]);
+      S[(size_t)kPacked * cap + i] = pack_slot((uint32_t)x, s.mode[x], s.phase[x], true);
+    }
+    SimHdr h{};
+    h.n = (int32_t)list.size();
+    img_hdr_[t] = h;
+  }
+  // Per-simulation layout.
+  desc_h_.resize(nsims_);
+  std::vector<int32_t> tumor_of(nsims_);
+  for (int i = 0; i < nsims_; ++i) {
+    const int t = spec.tumor_index[i];
+    EMT_ENFORCE(t >= 0 && t < T, -1, "Tumour index out of range: ", t);
+    EMT_ENFORCE(spec.prot_index[i] >= 0 && spec.prot_index[i] < spec.n_prot, -1,
+                "Protocol index out of range: ", spec.prot_index[i]);
+    const TumorState &s = *tumors[t];
+    SimDesc d{};
+    d.sub_off = sub_total_;
+    d.st_off = st_total_;
+    d.slot_off = slot_total_;
+    d.H = s.H;
+    d.W = s.W;
+    d.cap = slot_cap(s.H, s.W);
+    d.prot = spec.prot_index[i];
+    d.sim_id = spec.sim_id[i];
+    d.horizon = spec.horizon[i];
+    desc_h_[i] = d;
+    tumor_of[i] = t;
+    sub_total_ += 3 * (int64_t)s.sites();
+    st_total_ += st_bytes(s.H, s.W);
+    slot_total_ += 2 * (int64_t)kSlotFields * d.cap;
+    hmax_ = std::max(hmax_, s.H);
+    wmax_ = std::max(wmax_, s.W);
+    capmax_ = std::max(capmax_, d.cap);
+  }
+  // Longest work first: lattice area times horizon.
+  order_h_.resize(nsims_);
+  std::iota(order_h_.begin(), order_h_.end(), 0);
+  std::stable_sort(order_h_.begin(), order_h_.end(), [&](int a, int b) {
+    const double ca = (double)desc_h_[a].H * desc_h_[a].W * desc_h_[a].horizon;
+    const double cb = (double)desc_h_[b].H * desc_h_[b].W * desc_h_[b].horizon;
+    return ca > cb;
+  });
+  EMT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  EMT_HIP(hipEventCreate(&ev0_));
+  EMT_HIP(hipEventCreate(&ev1_));
+  desc_ = dalloc<SimDesc>(nsims_);
+  hdr_ = dalloc<SimHdr>(nsims_);
+  sub_ = dalloc<float>(sub_total_);
+  st_ = dalloc<uint8_t>(st_total_);
+  slots_ = dalloc<uint32_t>(slot_total_);
+  prot_ = dalloc<float>((size_t)spec.n_prot * spec.n_slots);
+  order_ = dalloc<int32_t>(nsims_);
+  queue_ = dalloc<int32_t>(1);
+  results_ = dalloc<uint32_t>(nsims_);
+  EMT_HIP(hipMemcpyAsync(desc_, desc_h_.data(), nsims_ * sizeof(SimDesc), hipMemcpyHostToDevice, stream_));
+  EMT_HIP(hipMemcpyAsync(order_, order_h_.data(), nsims_ * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
+  EMT_HIP(hipMemsetAsync(st_, 0, st_total_, stream_));
+  EMT_HIP(hipMemsetAsync(prot_, 0, (size_t)spec.n_prot * spec.n_slots * sizeof(float), stream_));
+  // Images on the device (kept for reset).
+  float *d_isub = dalloc<float>(isub);
+  uint8_t *d_ist = dalloc<uint8_t>(ist);
+  uint32_t *d_islot = dalloc<uint32_t>(islot);
+  ImgDesc *d_img = dalloc<ImgDesc>(T);
+  SimHdr *d_ihdr = dalloc<SimHdr>(T);
+  int32_t *d_tumor_of = dalloc<int32_t>(nsims_);
+  EMT_HIP(hipMemcpyAsync(d_isub, h_sub.data(), isub * sizeof(float), hipMemcpyHostToDevice, stream_));
+  EMT_HIP(hipMemcpyAsync(d_ist, h_st.data(), ist, hipMemcpyHostToDevice, stream_));
+  EMT_HIP(hipMemcpyAsync(d_islot, h_slots.data(), islot * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+  EMT_HIP(hipMemcpyAsync(d_img, img.data(), T * sizeof(ImgDesc), hipMemcpyHostToDevice, stream_));
+  EMT_HIP(hipMemcpyAsync(d_ihdr, img_hdr_.data(), T * sizeof(SimHdr), hipMemcpyHostToDevice, stream_));
+  EMT_HIP(hipMemcpyAsync(d_tumor_of, tumor_of.data(), nsims_ * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
+  img_sub_ = {d_isub};
+  img_st_ = {d_ist};
+  img_slots_ = {d_islot};
+  // Keep the small tables with the images.
+  img_desc_ = d_img;
+  img_hdr_d_ = d_ihdr;
+  tumor_of_ = d_tumor_of;
+  sync();
+  reset(0);
+}
+
+Batch::~Batch() {
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  dfree(desc_); dfree(hdr_); dfree(sub_); dfree(st_); dfree(slots_); dfree(prot_);
+  dfree(order_); dfree(queue_); dfree(results_); dfree(scratch_); dfree(ovf_);
+  for (auto &p : img_sub_) dfree(p);
+  for (auto &p : img_st_) dfree(p);
+  for (auto &p : img_slots_) dfree(p);
+  dfree(img_desc_); dfree(img_hdr_d_); dfree(tumor_of_);
+  if (ev0_) (void)hipEventDestroy(ev0_);
+  if (ev1_) (void)hipEventDestroy(ev1_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Batch::sync() { EMT_HIP(hipStreamSynchronize(stream_)); }
+
+void Batch::set_protocols(const float *doses) {
+  EMT_HIP(hipSetDevice(device_));
+  EMT_HIP(hipMemcpyAsync(prot_, doses, (size_t)spec_.n_prot * spec_.n_slots * sizeof(float),
+                         hipMemcpyHostToDevice, stream_));
+  sync();
+}
+
+void Batch::reset(uint32_t epoch) {
+  EMT_HIP(hipSetDevice(device_));
+  epoch_ = epoch;
+  hipLaunchKernelGGL(replicate_kernel, dim3(nsims_), dim3(256), 0, stream_, desc_, tumor_of_,
+                     img_desc_, img_hdr_d_, img_sub_[0], img_st_[0], img_slots_[0], sub_, st_,
+                     slots_, hdr_, nsims_);
+  EMT_HIP(hipGetLastError());
+  sync();
+}
+
+void Batch::ensure_scratch(int wgs) {
+  if (wgs <= scratch_wgs_) return;
+  dfree(scratch_);
+  dfree(ovf_);
+  const int64_t sstride = 32 * (int64_t)hmax_;
+  const int64_t ostride = capmax_;
+  scratch_ = dalloc<int32_t>((size_t)wgs * sstride);
+  ovf_ = dalloc<uint64_t>((size_t)wgs * ostride);
+  scratch_wgs_ = wgs;
+}
+
+dev::RunArgs Batch::args(uint32_t nsteps) const {
+  dev::RunArgs a{};
+  a.p = dparams_;
+  a.desc = desc_;
+  a.hdr = hdr_;
+  a.sub_pool = sub_;
+  a.st_pool = st_;
+  a.slot_pool = slots_;
+  a.prot_data = prot_;
+  a.n_slots = spec_.n_slots;
+  a.prot_res = spec_.prot_res;
+  a.prot_length = spec_.prot_length;
+  a.seed = seed_;
+  a.epoch = epoch_;
+  a.nsteps = nsteps;
+  a.order = order_;
+  a.nsims = nsims_;
+  a.queue = queue_;
+  a.scratch = scratch_;
+  a.scratch_stride = 32 * (int64_t)hmax_;
+  a.ovf = ovf_;
+  a.ovf_stride = capmax_;
+  a.pv_max = (hmax_ + 2) * (wmax_ + 2);
+  return a;
+}
+
+int Batch::choose_variant() const {
+  int cnt = 0;
+  const VariantInfo *v = variant_table(&cnt);
+  for (int i = 0; i < cnt; ++i)
+    if (variant_fits(v[i], hmax_, wmax_)) return i;
+  return -1;
+}
+
+void Batch::run(uint32_t nsteps, int path, int variant, uint32_t steps_per_launch) {
+  EMT_HIP(hipSetDevice(device_));
+  int cnt = 0;
+  const VariantInfo *vt = variant_table(&cnt);
+  if (variant < 0) variant = choose_variant();
+  EMT_ENFORCE(variant >= 0 && variant < cnt, -1, "No kernel variant fits lattices of ", hmax_,
+              " x ", wmax_);
+  EMT_ENFORCE(variant_fits(vt[variant], hmax_, wmax_), -1, "Kernel variant ", variant,
+              " does not fit lattices of ", hmax_, " x ", wmax_);
+  int64_t launches = 0;
+  if (path == kPathPersistent) {
+    const size_t lds = persistent_lds_bytes(hmax_, wmax_);
+    int dev_cus = 0;
+    EMT_HIP(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, device_));
+    const int occ = persistent_occupancy(variant, lds);
+    EMT_ENFORCE(occ > 0, -1, "Persistent kernel does not fit: ", lds, " bytes of LDS");
+    const int grid = std::min(nsims_, occ * dev_cus);
+    ensure_scratch(grid);
+    EMT_HIP(hipEventRecord(ev0_, stream_));
+    uint32_t done = 0;
+    const uint32_t chunk = steps_per_launch ? steps_per_launch : nsteps;
+    while (done < nsteps) {
+      const uint32_t k = std::min(chunk, nsteps - done);
+      EMT_HIP(launch_fill_i32(queue_, 0, 1, stream_));
+      EMT_HIP(launch_persistent_lds(variant, args(k), grid, lds, stream_));
+      launches += 2;
+      done += k;
+    }
+    EMT_HIP(hipEventRecord(ev1_, stream_));
+  } else if (path == kPathStep || path == kPathStepGraph) {
+    ensure_scratch(nsims_);
+    const dev::RunArgs a = args(1);
+    EMT_HIP(hipEventRecord(ev0_, stream_));
+    uint32_t done = 0;
+    if (path == kPathStepGraph) {
+      const int K = 128;
+      if (!graph_exec_ || graph_variant_ != variant || graph_steps_ != K) {
+        if (graph_exec_) EMT_HIP(hipGraphExecDestroy(graph_exec_));
+        graph_exec_ = nullptr;
+        hipGraph_t g = nullptr;
+        EMT_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        for (int s = 0; s < K; ++s) EMT_HIP(launch_step(variant, a, nsims_, hmax_, wmax_, stream_));
+        EMT_HIP(hipStreamEndCapture(stream_, &g));
+        EMT_HIP(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0));
+        EMT_HIP(hipGraphDestroy(g));
+        graph_variant_ = variant;
+        graph_steps_ = K;
+      }
+      while (nsteps - done >= (uint32_t)K) {
+        EMT_HIP(hipGraphLaunch(graph_exec_, stream_));
+        launches += 3 * K;
+        done += K;
+      }
+    }
+    while (done < nsteps) {
+      EMT_HIP(launch_step(variant, a, nsims_, hmax_, wmax_, stream_));
+      launches += 3;
+      ++done;
+    }
+    EMT_HIP(hipEventRecord(ev1_, stream_));
+  } else {
+    throw Error(cat("Check failed: valid path; unknown run path ", path), -1);
+  }
+  sync();
+  float ms = 0.f;
+  EMT_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+  last_ms_ = ms;
+  last_launches_ = launches;
+}
+
+void Batch::results(uint32_t *dst) {
+  EMT_HIP(hipSetDevice(device_));
+  EMT_HIP(launch_count_living(desc_, st_, results_, nsims_, stream_));
+  EMT_HIP(hipMemcpyAsync(dst, results_, nsims_ * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+  sync();
+}
+
+void Batch::counters(uint64_t *dst) {
+  EMT_HIP(hipSetDevice(device_));
+  std::vector<SimHdr> h(nsims_);
+  EMT_HIP(hipMemcpyAsync(h.data(), hdr_, nsims_ * sizeof(SimHdr), hipMemcpyDeviceToHost, stream_));
+  sync();
+  for (int i = 0; i < nsims_; ++i) {
+    dst[5 * i + 0] = h[i].step;
+    dst[5 * i + 1] = (uint64_t)h[i].n;
+    dst[5 * i + 2] = h[i].divisions;
+    dst[5 * i + 3] = h[i].status;
+    dst[5 * i + 4] = h[i].site_substeps;
+  }
+}
+
+TumorState Batch::state(int sim) {
+  EMT_ENFORCE(sim >= 0 && sim < nsims_, -1, "Cannot read data from sample ", sim,
+              ". Batch size: ", nsims_);
+  EMT_HIP(hipSetDevice(device_));
+  const SimDesc &d = desc_h_[sim];
+  TumorState s(d.H, d.W);
+  const size_t hw = s.sites();
+  std::vector<float> sub(3 * hw);
+  std::vector<uint32_t> slots((size_t)2 * kSlotFields * d.cap);
+  SimHdr h{};
+  EMT_HIP(hipMemcpyAsync(sub.data(), sub_ + d.sub_off, 3 * hw * sizeof(float), hipMemcpyDeviceToHost, stream_));
+  EMT_HIP(hipMemcpyAsync(s.occ.data(), st_ + d.st_off, hw, hipMemcpyDeviceToHost, stream_));
+  EMT_HIP(hipMemcpyAsync(slots.data(), slots_ + d.slot_off, slots.size() * 4, hipMemcpyDeviceToHost, stream_));
+  EMT_HIP(hipMemcpyAsync(&h, hdr_ + sim, sizeof(SimHdr), hipMemcpyDeviceToHost, stream_));
+  sync();
+  for (int k = 0; k < 3; ++k) std::memcpy(s.f[k].data(), &sub[k * hw], hw * sizeof(float));
+  const uint32_t *S = &slots[(size_t)h.cur * kSlotFields * d.cap];
+  for (int i = 0; i < h.n; ++i) {
+    const uint32_t w = S[(size_t)kPacked * d.cap + i];
+    if (!slot_live(w)) continue;
+    const uint32_t x = slot_site(w);
+    auto get = [&](int field) { float v; std::memcpy(&v, &S[(size_t)field * d.cap + i], 4); return v; };
+    s.f[TumorState::kTRep][x] = get(kTRep);
+    s.f[TumorState::kIrr][x] = get(kR);
+    s.f[TumorState::kDelay][x] = get(kDelay);
+    s.f[TumorState::kProlif][x] = get(kProlif);
+    for (int k = 0; k < 5; ++k) s.f[TumorState::kG1 + k][x] = get(kT0 + k);
+    s.mode[x] = (uint8_t)slot_mode(w);
+    s.phase[x] = (uint8_t)slot_phase(w);
+  }
+  return s;
+}
+
+}  // namespace emt6ro

@@ -1,0 +1,397 @@
+// EMT6/Ro simulation kernels for gfx950.
+//
+//   persistent_kernel : one workgroup per simulation at a time, pulled from a
+//                       work queue; the padded lattice substrates and site
+//                       states stay in LDS for every step of the run, the
+//                       cell slots stay in L2-resident global memory.
+//   roi_kernel, diffuse_kernel, cells_kernel : the per-step path with the
+//                       reference's structure (simulation.cu:153-180): one
+//                       launch per phase per step, state in HBM between them.
+//   count_living_kernel : living cells per simulation (statistics.cu:9-38).
+#include <hip/hip_runtime.h>
+#include "kernels.h"
+#include "sim_phases.h"
+
+namespace emt6ro {
+namespace dev {
+
+// Loads a simulation's substrates into the padded LDS planes (zero ring) and
+// its site states into LDS.
+template <int NT>
+__device__ void load_lattice(const RunArgs &a, const SimDesc &d, float *sub, uint8_t *st) {
+  const int H = d.H, W = d.W, PW = W + 2, PV = (H + 2) * PW, HW = H * W;
+  const float *g = a.sub_pool + d.sub_off;
+  for (int q = threadIdx.x; q < PV; q += NT) {
+    const int pr = q / PW, pc = q - pr * PW;
+    const bool in = pr >= 1 && pr <= H && pc >= 1 && pc <= W;
+    const int x = (pr - 1) * W + (pc - 1);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) sub[s * PV + q] = in ? g[s * HW + x] : 0.f;
+  }
+  const uint32_t *gs = reinterpret_cast<const uint32_t *>(a.st_pool + d.st_off);
+  uint32_t *ls = reinterpret_cast<uint32_t *>(st);
+  for (int k = threadIdx.x; k < (HW + 3) / 4; k += NT) ls[k] = gs[k];
+}
+
+template <int NT>
+__device__ void store_lattice(const RunArgs &a, const SimDesc &d, const float *sub,
+                              const uint8_t *st) {
+  const int H = d.H, W = d.W, PW = W + 2, PV = (H + 2) * PW, HW = H * W;
+  float *g = a.sub_pool + d.sub_off;
+  for (int x = threadIdx.x; x < HW; x += NT) {
+    const int r = x / W, c = x - r * W;
+    const int q = (r + 1) * PW + (c + 1);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) g[s * HW + x] = sub[s * PV + q];
+  }
+  uint32_t *gs = reinterpret_cast<uint32_t *>(a.st_pool + d.st_off);
+  const uint32_t *ls = reinterpret_cast<const uint32_t *>(st);
+  for (int k = threadIdx.x; k < (HW + 3) / 4; k += NT) gs[k] = ls[k];
+}
+
+template <int NT, int L, int KMAX, int PS>
+__global__ __launch_bounds__(NT) void persistent_kernel(RunArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  Shared &sm = *reinterpret_cast<Shared *>(smem);
+  float *sub = reinterpret_cast<float *>(smem + sizeof(Shared));
+  uint8_t *st = reinterpret_cast<uint8_t *>(sub + 3 * a.pv_max);
+  int32_t *pfx = a.scratch + (int64_t)blockIdx.x * a.scratch_stride;
+  uint64_t *ovf = a.ovf + (int64_t)blockIdx.x * a.ovf_stride;
+  const int tid = threadIdx.x;
+  for (;;) {
+    if (tid == 0) sm.sim = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int qi = sm.sim;
+    if (qi >= a.nsims) break;
+    const int sim = a.order[qi];
+    const SimDesc d = a.desc[sim];
+    SimHdr *hd = a.hdr + sim;
+    const uint32_t step0 = hd->step;
+    const uint32_t left = d.horizon > step0 ? d.horizon - step0 : 0u;
+    const uint32_t nsteps = a.nsteps < left ? a.nsteps : left;
+    if (nsteps == 0u) {
+      __syncthreads();
+      continue;
+    }
+    const int H = d.H, W = d.W, PW = W + 2, PV = (H + 2) * PW;
+    load_lattice<NT>(a, d, sub, st);
+    if (tid == 0) {
+      sm.n = hd->n;
+      sm.cur = hd->cur;
+      for (int k = 0; k < 5; ++k) sm.roi[k] = hd->roi[k];
+      sm.divisions = 0u;
+      sm.status = 0u;
+    }
+    __syncthreads();
+    uint32_t *slots = a.slot_pool + d.slot_off;
+    const RngKey key{a.seed, a.epoch, d.sim_id};
+    uint32_t updates = 0u;
+    uint64_t updates64 = 0u;
+    for (uint32_t k = 0; k < nsteps; ++k) {
+      const uint32_t t = step0 + k;
+      if (t % 128u == 0u) rebuild_list<NT>(st, H, W, slots, d.cap, pfx, sm);
+      if (t % 32u == 0u) find_roi<NT>(st, H, W, sm);
+      int roi[5];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) roi[q] = sm.roi[q];
+      diffuse<NT, L, KMAX, PS>(a.p, sub, PW, PV, roi, updates);
+      const float dose = dose_at(a, d.prot, t);
+      cells_step<NT>(a.p, key, t, dose, SubLds{sub, PW, PV}, st, H, W, slots, d.cap, ovf, sm);
+      if ((k & 1023u) == 1023u) { updates64 += updates; updates = 0u; }
+    }
+    updates64 += updates;
+    store_lattice<NT>(a, d, sub, st);
+    const uint64_t tot = wg_reduce_sum_u64<NT>(updates64, sm.red);
+    if (tid == 0) {
+      hd->step = step0 + nsteps;
+      hd->n = sm.n;
+      hd->cur = sm.cur;
+      for (int q = 0; q < 5; ++q) hd->roi[q] = sm.roi[q];
+      hd->divisions += sm.divisions;
+      hd->status |= sm.status;
+      hd->site_substeps += tot;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-step path.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool sim_active(const RunArgs &a, int sim) {
+  return a.hdr[sim].step < a.desc[sim].horizon;
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void roi_kernel(RunArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  Shared &sm = *reinterpret_cast<Shared *>(smem);
+  uint8_t *st = smem + sizeof(Shared);
+  const int sim = blockIdx.x;
+  if (!sim_active(a, sim) || a.hdr[sim].step % 32u != 0u) return;
+  const SimDesc d = a.desc[sim];
+  const int HW = d.H * d.W;
+  const uint32_t *gs = reinterpret_cast<const uint32_t *>(a.st_pool + d.st_off);
+  uint32_t *ls = reinterpret_cast<uint32_t *>(st);
+  for (int k = threadIdx.x; k < (HW + 3) / 4; k += NT) ls[k] = gs[k];
+  __syncthreads();
+  find_roi<NT>(st, d.H, d.W, sm);
+  if (threadIdx.x < 5) a.hdr[sim].roi[threadIdx.x] = sm.roi[threadIdx.x];
+}
+
+template <int NT, int L, int KMAX, int PS>
+__global__ __launch_bounds__(NT) void diffuse_kernel(RunArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  Shared &sm = *reinterpret_cast<Shared *>(smem);
+  float *sub = reinterpret_cast<float *>(smem + sizeof(Shared));
+  const int sim = blockIdx.x;
+  if (!sim_active(a, sim)) return;
+  const SimDesc d = a.desc[sim];
+  int roi[5];
+  for (int q = 0; q < 5; ++q) roi[q] = a.hdr[sim].roi[q];
+  if (roi[2] == 0 || roi[3] == 0) return;
+  const int H = d.H, W = d.W, PW = W + 2, PV = (H + 2) * PW, HW = H * W;
+  // Temporary grid: padded rows r0-1 .. r0+h+2, cols c0-1 .. c0+w+2.
+  const int R0 = roi[0] - 1, C0 = roi[1] - 1, th = roi[2] + 4, tw = roi[3] + 4;
+  const float *g = a.sub_pool + d.sub_off;
+  for (int q = threadIdx.x; q < th * tw; q += NT) {
+    const int pr = R0 + q / tw, pc = C0 + q % tw;
+    const bool in = pr >= 1 && pr <= H && pc >= 1 && pc <= W;
+    const int x = (pr - 1) * W + (pc - 1);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) sub[s * PV + pr * PW + pc] = in ? g[s * HW + x] : 0.f;
+  }
+  (void)sm;
+  uint32_t updates = 0u;
+  diffuse<NT, L, KMAX, PS>(a.p, sub, PW, PV, roi, updates);
+  float *gw = a.sub_pool + d.sub_off;
+  for (int q = threadIdx.x; q < roi[2] * roi[3]; q += NT) {
+    const int r = roi[0] + q / roi[3], c = roi[1] + q % roi[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) gw[s * HW + r * W + c] = sub[s * PV + (r + 1) * PW + (c + 1)];
+  }
+  const uint64_t tot = wg_reduce_sum_u64<NT>(updates, sm.red);
+  if (threadIdx.x == 0) atomicAdd((unsigned long long *)&a.hdr[sim].site_substeps,
+                                  (unsigned long long)tot);
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void cells_kernel(RunArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  Shared &sm = *reinterpret_cast<Shared *>(smem);
+  uint8_t *st = smem + sizeof(Shared);
+  const int sim = blockIdx.x;
+  if (!sim_active(a, sim)) return;
+  const SimDesc d = a.desc[sim];
+  SimHdr *hd = a.hdr + sim;
+  const int H = d.H, W = d.W, HW = H * W;
+  uint32_t *gs = reinterpret_cast<uint32_t *>(a.st_pool + d.st_off);
+  uint32_t *ls = reinterpret_cast<uint32_t *>(st);
+  for (int k = threadIdx.x; k < (HW + 3) / 4; k += NT) ls[k] = gs[k];
+  const uint32_t t = hd->step;
+  if (threadIdx.x == 0) {
+    sm.n = hd->n;
+    sm.cur = hd->cur;
+    sm.divisions = 0u;
+    sm.status = 0u;
+  }
+  __syncthreads();
+  uint32_t *slots = a.slot_pool + d.slot_off;
+  int32_t *pfx = a.scratch + (int64_t)blockIdx.x * a.scratch_stride;
+  uint64_t *ovf = a.ovf + (int64_t)blockIdx.x * a.ovf_stride;
+  if (t % 128u == 0u) rebuild_list<NT>(st, H, W, slots, d.cap, pfx, sm);
+  const RngKey key{a.seed, a.epoch, d.sim_id};
+  const float dose = dose_at(a, d.prot, t);
+  cells_step<NT>(a.p, key, t, dose, SubGlobal{a.sub_pool + d.sub_off, W, HW}, st, H, W, slots,
+                 d.cap, ovf, sm);
+  for (int k = threadIdx.x; k < (HW + 3) / 4; k += NT) gs[k] = ls[k];
+  if (threadIdx.x == 0) {
+    hd->step = t + 1u;
+    hd->n = sm.n;
+    hd->cur = sm.cur;
+    hd->divisions += sm.divisions;
+    hd->status |= sm.status;
+  }
+}
+
+// countLiving (statistics.cu:9-38)
+template <int NT>
+__global__ __launch_bounds__(NT) void count_living_kernel(const SimDesc *desc,
+                                                          const uint8_t *st_pool,
+                                                          uint32_t *out, int nsims) {
+  __shared__ int32_t red[NT / 64];
+  const int sim = blockIdx.x;
+  if (sim >= nsims) return;
+  const SimDesc d = desc[sim];
+  const uint8_t *st = st_pool + d.st_off;
+  int cnt = 0;
+  for (int x = threadIdx.x; x < d.H * d.W; x += NT) cnt += (st[x] & 3) == kOccupied;
+  for (int m = 32; m >= 1; m >>= 1) cnt += __shfl_xor(cnt, m, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int w = 0; w < NT / 64; ++w) s += red[w];
+    out[sim] = (uint32_t)s;
+  }
+}
+
+__global__ void fill_i32_kernel(int32_t *p, int32_t v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace dev
+
+// ---------------------------------------------------------------------------
+// Host-side launchers.
+// ---------------------------------------------------------------------------
+namespace {
+
+size_t shared_bytes() { return sizeof(dev::Shared); }
+
+template <int NT, int L, int KMAX, int PS>
+hipError_t launch_persistent_t(const dev::RunArgs &a, int grid, size_t lds, hipStream_t s) {
+  auto k = dev::persistent_kernel<NT, L, KMAX, PS>;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3(grid), dim3(NT), lds, s, a);
+  return hipGetLastError();
+}
+
+template <int NT, int L, int KMAX, int PS>
+hipError_t launch_diffuse_t(const dev::RunArgs &a, int grid, size_t lds, hipStream_t s) {
+  auto k = dev::diffuse_kernel<NT, L, KMAX, PS>;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3(grid), dim3(NT), lds, s, a);
+  return hipGetLastError();
+}
+
+template <int NT, int L, int KMAX, int PS>
+int variant_occupancy(size_t lds) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &nb, reinterpret_cast<const void *>(dev::persistent_kernel<NT, L, KMAX, PS>), NT,
+          lds) != hipSuccess)
+    return 0;
+  return nb;
+}
+
+}  // namespace
+
+// Kernel variants: workgroup size, diffusion band height, items per thread and
+// ring-save slots per thread.
+#define EMT6RO_VARIANTS(X)  \
+  X(0, 256, 4, 3, 2)        \
+  X(1, 256, 2, 6, 2)        \
+  X(2, 256, 8, 2, 2)        \
+  X(3, 512, 4, 2, 1)        \
+  X(4, 512, 8, 1, 1)        \
+  X(5, 128, 4, 6, 4)        \
+  X(6, 256, 4, 7, 3)        \
+  X(7, 512, 4, 4, 2)
+
+const VariantInfo *variant_table(int *count) {
+  static const VariantInfo table[] = {
+#define X(id, nt, l, kmax, ps) {id, nt, l, kmax, ps},
+      EMT6RO_VARIANTS(X)
+#undef X
+  };
+  *count = (int)(sizeof(table) / sizeof(table[0]));
+  return table;
+}
+
+size_t persistent_lds_bytes(int hmax, int wmax) {
+  const size_t pv = (size_t)(hmax + 2) * (wmax + 2);
+  const size_t stb = ((size_t)hmax * wmax + 15) & ~(size_t)15;
+  return shared_bytes() + 3 * pv * sizeof(float) + stb;
+}
+
+size_t diffuse_lds_bytes(int hmax, int wmax) {
+  const size_t pv = (size_t)(hmax + 2) * (wmax + 2);
+  return shared_bytes() + 3 * pv * sizeof(float);
+}
+
+size_t cells_lds_bytes(int hmax, int wmax) {
+  return shared_bytes() + (((size_t)hmax * wmax + 15) & ~(size_t)15);
+}
+
+hipError_t launch_persistent_lds(int variant, const dev::RunArgs &a, int grid, size_t lds,
+                                 hipStream_t s) {
+  switch (variant) {
+#define X(id, nt, l, kmax, ps) \
+  case id: return launch_persistent_t<nt, l, kmax, ps>(a, grid, lds, s);
+    EMT6RO_VARIANTS(X)
+#undef X
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int persistent_occupancy(int variant, size_t lds) {
+  switch (variant) {
+#define X(id, nt, l, kmax, ps) \
+  case id: return variant_occupancy<nt, l, kmax, ps>(lds);
+    EMT6RO_VARIANTS(X)
+#undef X
+    default: return 0;
+  }
+}
+
+hipError_t launch_step(int variant, const dev::RunArgs &a, int nsims, int hmax, int wmax,
+                       hipStream_t s) {
+  int cnt = 0;
+  const VariantInfo *v = variant_table(&cnt);
+  if (variant < 0 || variant >= cnt) return hipErrorInvalidValue;
+  const int nt = v[variant].nt;
+  // ROI (every 32 steps, decided per simulation inside the kernel).
+  {
+    const size_t lds = cells_lds_bytes(hmax, wmax);
+    if (nt == 512)
+      hipLaunchKernelGGL(dev::roi_kernel<512>, dim3(nsims), dim3(512), lds, s, a);
+    else if (nt == 128)
+      hipLaunchKernelGGL(dev::roi_kernel<128>, dim3(nsims), dim3(128), lds, s, a);
+    else
+      hipLaunchKernelGGL(dev::roi_kernel<256>, dim3(nsims), dim3(256), lds, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  {
+    const size_t lds = diffuse_lds_bytes(hmax, wmax);
+    hipError_t e = hipErrorInvalidValue;
+    switch (variant) {
+#define X(id, nt_, l, kmax, ps) \
+  case id: e = launch_diffuse_t<nt_, l, kmax, ps>(a, nsims, lds, s); break;
+      EMT6RO_VARIANTS(X)
+#undef X
+      default: break;
+    }
+    if (e != hipSuccess) return e;
+  }
+  {
+    const size_t lds = cells_lds_bytes(hmax, wmax);
+    if (nt == 512)
+      hipLaunchKernelGGL(dev::cells_kernel<512>, dim3(nsims), dim3(512), lds, s, a);
+    else if (nt == 128)
+      hipLaunchKernelGGL(dev::cells_kernel<128>, dim3(nsims), dim3(128), lds, s, a);
+    else
+      hipLaunchKernelGGL(dev::cells_kernel<256>, dim3(nsims), dim3(256), lds, s, a);
+    return hipGetLastError();
+  }
+}
+
+hipError_t launch_count_living(const SimDesc *desc, const uint8_t *st_pool, uint32_t *out,
+                               int nsims, hipStream_t s) {
+  hipLaunchKernelGGL(dev::count_living_kernel<256>, dim3(nsims), dim3(256), 0, s, desc, st_pool,
+                     out, nsims);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int32_t *p, int32_t v, int n, hipStream_t s) {
+  hipLaunchKernelGGL(dev::fill_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+}  // namespace emt6ro

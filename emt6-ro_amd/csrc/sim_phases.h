@@ -1,0 +1,599 @@
+// The phases of one EMT6/Ro step as workgroup-wide device functions: occupied
+// list rebuild, region of interest, substrate diffusion and the cell update.
+// The persistent kernel calls them with the lattice resident in LDS; the
+// per-step kernels wrap each phase with its own loads and stores.
+//
+// One workgroup of NT threads owns one simulation.  Every function here is
+// entered by all NT threads with uniform arguments and contains its own
+// barriers.
+#ifndef EMT6RO_AMD_SIM_PHASES_H_
+#define EMT6RO_AMD_SIM_PHASES_H_
+
+#include "device.h"
+
+namespace emt6ro {
+namespace dev {
+
+// Ready-to-divide entries kept in LDS per step; more spill to global memory.
+constexpr int kReadyLds = 64;
+constexpr uint32_t kNoWinner = 0xffffffffu;
+
+// Workgroup scalars in LDS.
+struct alignas(16) Shared {
+  uint64_t ready[kReadyLds];  // slot | child << 16 | R bits << 32
+  int32_t n, cur, ready_cnt, sim;
+  int32_t roi[5];
+  uint32_t winner, divisions, status;
+  int32_t red[64];            // reduction scratch (up to 16 waves x 4 values)
+};
+
+// Kernel-wide arguments shared by all kernels of a run.
+struct RunArgs {
+  DevParams p;
+  const SimDesc *desc;
+  SimHdr *hdr;
+  float *sub_pool;
+  uint8_t *st_pool;
+  uint32_t *slot_pool;
+  const float *prot_data;     // n_prot rows of n_slots doses
+  int32_t n_slots;
+  uint32_t prot_res, prot_length;
+  uint32_t seed, epoch;
+  uint32_t nsteps;            // steps to advance each simulation (capped by horizon)
+  const int32_t *order;       // simulation processing order (persistent kernel)
+  int32_t nsims;
+  int32_t *queue;             // work counter (persistent kernel)
+  int32_t *scratch;           // per-workgroup prefix table, scratch_stride ints
+  int64_t scratch_stride;
+  uint64_t *ovf;              // per-workgroup ready-entry spill, ovf_stride entries
+  int64_t ovf_stride;
+  int32_t pv_max;             // LDS floats per substrate plane
+};
+
+// Protocol::getDose (protocol.h:9-14)
+__device__ __forceinline__ float dose_at(const RunArgs &a, int prot, uint32_t t) {
+  if (t < a.prot_length && t % a.prot_res == 0u) {
+    const uint32_t slot = t / a.prot_res;
+    if ((int32_t)slot < a.n_slots) return a.prot_data[(int64_t)prot * a.n_slots + slot];
+  }
+  return 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup reductions (64-wide wavefronts).
+// ---------------------------------------------------------------------------
+template <int NT>
+__device__ __forceinline__ int wg_reduce_min(int v, int32_t *red) {
+  for (int m = 32; m >= 1; m >>= 1) { int y = __shfl_xor(v, m, 64); v = y < v ? y : v; }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int r = red[0];
+  for (int w = 1; w < NT / 64; ++w) r = red[w] < r ? red[w] : r;
+  __syncthreads();
+  return r;
+}
+
+template <int NT>
+__device__ __forceinline__ int wg_reduce_max(int v, int32_t *red) {
+  for (int m = 32; m >= 1; m >>= 1) { int y = __shfl_xor(v, m, 64); v = y > v ? y : v; }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int r = red[0];
+  for (int w = 1; w < NT / 64; ++w) r = red[w] > r ? red[w] : r;
+  __syncthreads();
+  return r;
+}
+
+// Four minima at once: (a, b, -c, -d) reduced together.
+template <int NT>
+__device__ __forceinline__ void wg_reduce_min4(int v[4], int32_t *red) {
+  for (int m = 32; m >= 1; m >>= 1)
+    for (int k = 0; k < 4; ++k) { int y = __shfl_xor(v[k], m, 64); v[k] = y < v[k] ? y : v[k]; }
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 4; ++k) red[(threadIdx.x >> 6) * 4 + k] = v[k];
+  __syncthreads();
+  for (int k = 0; k < 4; ++k) {
+    int r = red[k];
+    for (int w = 1; w < NT / 64; ++w) r = red[w * 4 + k] < r ? red[w * 4 + k] : r;
+    v[k] = r;
+  }
+  __syncthreads();
+}
+
+template <int NT>
+__device__ __forceinline__ uint64_t wg_reduce_sum_u64(uint64_t v, int32_t *red) {
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  uint64_t *r64 = reinterpret_cast<uint64_t *>(red);
+  if ((threadIdx.x & 63) == 0) r64[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint64_t r = 0;
+  for (int w = 0; w < NT / 64; ++w) r += r64[w];
+  __syncthreads();
+  return r;
+}
+
+// Exclusive prefix sum over the workgroup; returns the offset, sets total.
+template <int NT>
+__device__ __forceinline__ int wg_exclusive_scan(int v, int32_t *red, int &total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int inc = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    int y = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += y;
+  }
+  if (lane == 63) red[wid] = inc;
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int w = 0; w < NT / 64; ++w) {
+    int s = red[w];
+    off += (w < wid) ? s : 0;
+    tot += s;
+  }
+  total = tot;
+  __syncthreads();
+  return off + inc - v;
+}
+
+// ---------------------------------------------------------------------------
+// Occupied-list rebuild (findOccupied, simulation.cu:41-64).  The reference
+// scans with 32 threads: thread g takes columns g, g+32, g+64, ... of rows
+// 0..H-2, and the list is the concatenation of the threads' lists.  The new
+// list order is written as a gather of the live slots into the other slot
+// buffer; `pfx` is a per-workgroup table of 32*(H-1) ints in global memory.
+// ---------------------------------------------------------------------------
+template <int NT>
+__device__ void rebuild_list(const uint8_t *st, int H, int W, uint32_t *slots, int cap,
+                             int32_t *pfx, Shared &sm) {
+  const int tid = threadIdx.x;
+  const int rows = H - 1;
+  const int npairs = 32 * rows;
+  const int per = (npairs + NT - 1) / NT;
+  const int q0 = tid * per;
+  const int q1 = min(q0 + per, npairs);
+  int local = 0;
+  for (int q = q0; q < q1; ++q) {
+    const int g = q / rows, r = q - g * rows;
+    for (int c = g; c < W - 1; c += 32) local += (st[r * W + c] & 3) == kOccupied;
+  }
+  int total;
+  int run = wg_exclusive_scan<NT>(local, sm.red, total);
+  for (int q = q0; q < q1; ++q) {
+    const int g = q / rows, r = q - g * rows;
+    pfx[q] = run;
+    for (int c = g; c < W - 1; c += 32) run += (st[r * W + c] & 3) == kOccupied;
+  }
+  __syncthreads();
+  const int n_old = sm.n;
+  const int cur = sm.cur;
+  const uint32_t *src = slots + (int64_t)cur * kSlotFields * cap;
+  uint32_t *dst = slots + (int64_t)(cur ^ 1) * kSlotFields * cap;
+  for (int i = tid; i < n_old; i += NT) {
+    const uint32_t w = src[(int64_t)kPacked * cap + i];
+    if (!slot_live(w)) continue;
+    const int x = (int)slot_site(w);
+    const int r = x / W, c = x - r * W;
+    const int g = c & 31;
+    int pos = pfx[g * rows + r];
+    for (int cc = g; cc < c; cc += 32) pos += (st[r * W + cc] & 3) == kOccupied;
+#pragma unroll
+    for (int f = 0; f < kSlotFields; ++f)
+      dst[(int64_t)f * cap + pos] = src[(int64_t)f * cap + i];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    sm.n = total;
+    sm.cur = cur ^ 1;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Region of interest (findROI, diffusion.cu:65-100): bounding box of the
+// occupied sites, its centre, the largest ceil(Euclidean distance) from the
+// centre to an occupied site, and the square of that half-width clipped to
+// the interior.  ceil(sqrtf(n)) of the reference equals the integer ceiling
+// square root for every n a lattice can produce.
+// ---------------------------------------------------------------------------
+template <int NT>
+__device__ void find_roi(const uint8_t *st, int H, int W, Shared &sm) {
+  const int tid = threadIdx.x;
+  int v[4] = {H, W, 0, 0};  // minr, minc, -maxr, -maxc
+  for (int x = tid; x < H * W; x += NT) {
+    if ((st[x] & 3) == kOccupied) {
+      const int r = x / W, c = x - r * W;
+      v[0] = min(v[0], r);
+      v[1] = min(v[1], c);
+      v[2] = min(v[2], -r);
+      v[3] = min(v[3], -c);
+    }
+  }
+  wg_reduce_min4<NT>(v, sm.red);
+  const int minr = v[0], minc = v[1], maxr = -v[2], maxc = -v[3];
+  if (maxr == 0 && maxc == 0) {
+    if (tid == 0) for (int k = 0; k < 5; ++k) sm.roi[k] = 0;
+    __syncthreads();
+    return;
+  }
+  const int midr = minr + (maxr - minr + 1) / 2;
+  const int midc = minc + (maxc - minc + 1) / 2;
+  int d2 = 0;
+  for (int x = tid; x < H * W; x += NT) {
+    if ((st[x] & 3) == kOccupied) {
+      const int r = x / W, c = x - r * W;
+      const int e = (midr - r) * (midr - r) + (midc - c) * (midc - c);
+      d2 = max(d2, e);
+    }
+  }
+  d2 = wg_reduce_max<NT>(d2, sm.red);
+  int dist = (int)sqrtf((float)d2);
+  while (dist * dist < d2) ++dist;
+  while (dist > 0 && (dist - 1) * (dist - 1) >= d2) --dist;
+  if (tid == 0) {
+    const int sr = (midr <= dist) ? 1 : midr - dist;
+    const int sc = (midc <= dist) ? 1 : midc - dist;
+    const int sw = (midc + dist >= W - 1) ? W - 1 - sc : midc + dist - sc + 1;
+    const int sh = (midr + dist >= H - 1) ? H - 1 - sr : midr + dist - sr + 1;
+    sm.roi[0] = sr; sm.roi[1] = sc; sm.roi[2] = sh; sm.roi[3] = sw; sm.roi[4] = dist;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Diffusion (diffusionKernel, diffusion.cu:119-191), in place on a padded
+// lattice in LDS: plane s of substrate s holds lattice site (r, c) at
+// (r+1)*PW + (c+1), with PW = W+2 and a zero ring around the lattice.
+//
+// The reference copies the ROI plus a two-site ring into a temporary grid:
+// outer ring 0, masked sites the external levels, other sites the lattice
+// values; it runs S-1 Jacobi substeps on the unmasked sites of the ROI plus
+// its inner ring ("mask box"), then one substep on every ROI site, and writes
+// the ROI back.  Here the two rings are saved to registers, set to their
+// temporary values, and restored afterwards, which gives the same ROI values
+// without a second grid.
+//
+// Mask: site (i, j) of the (h+2) x (w+2) mask box is masked when
+// sqrtf((i+1-(h+2)/2)^2 + (j+1-(w+2)/2)^2) >= dist (fillBorderMask,
+// diffusion.cu:16-30); all terms are multiples of 1/4, so this is the integer
+// test a^2 + b^2 >= 4 dist^2 with a = 2(i+1)-(h+2), b = 2(j+1)-(w+2).
+//
+// Work split: the mask box is cut into bands of L rows; an item is one
+// column of one band, and a thread walks its item down the rows with a
+// rolling 3x3 window, so lanes of a wave read consecutive LDS words.
+// Stencil, per substrate, in this order (no other fused operations):
+//   V(c) = up(c) + down(c),  o = V(c) + (left + right),  d = V(c-1) + V(c+1)
+//   L = fma(d, 1/sqrt2, o);  L = fma(-F, x, L);  x' = fma(k, L, x)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool masked(int i, int j, int hm, int wm, int d4) {
+  const int a = 2 * (i + 1) - hm, b = 2 * (j + 1) - wm;
+  return a * a + b * b >= d4;
+}
+
+template <int NT, int L, int KMAX, int PS>
+__device__ void diffuse(const DevParams &p, float *sub, int PW, int PV, const int roi[5],
+                        uint32_t &updates) {
+  const int r0 = roi[0], c0 = roi[1], h = roi[2], w = roi[3], dist = roi[4];
+  if (h == 0 || w == 0) return;
+  const int tid = threadIdx.x;
+  const int hm = h + 2, wm = w + 2;
+  const int d4 = 4 * dist * dist;
+  // Padded coordinates of the temporary grid: rows R0..R1, cols C0..C1.
+  const int R0 = r0 - 1, R1 = r0 + h + 2, C0 = c0 - 1, C1 = c0 + w + 2;
+  const int nper = 4 * w + 4 * h + 16;
+  float sv[PS][3];
+  int sidx[PS];
+#pragma unroll
+  for (int k = 0; k < PS; ++k) {
+    int q = tid + k * NT;
+    sidx[k] = -1;
+    if (q < nper) {
+      int pr, pc;
+      bool outer;
+      if (q < w + 4) { pr = R0; pc = C0 + q; outer = true; }
+      else if ((q -= w + 4) < w + 4) { pr = R1; pc = C0 + q; outer = true; }
+      else if ((q -= w + 4) < h + 2) { pr = R0 + 1 + q; pc = C0; outer = true; }
+      else if ((q -= h + 2) < h + 2) { pr = R0 + 1 + q; pc = C1; outer = true; }
+      else if ((q -= h + 2) < w + 2) { pr = R0 + 1; pc = C0 + 1 + q; outer = false; }
+      else if ((q -= w + 2) < w + 2) { pr = R1 - 1; pc = C0 + 1 + q; outer = false; }
+      else if ((q -= w + 2) < h) { pr = R0 + 2 + q; pc = C0 + 1; outer = false; }
+      else { q -= h; pr = R0 + 2 + q; pc = C1 - 1; outer = false; }
+      const int idx = pr * PW + pc;
+      sidx[k] = idx;
+#pragma unroll
+      for (int s = 0; s < 3; ++s) sv[k][s] = sub[s * PV + idx];
+      if (outer) {
+#pragma unroll
+        for (int s = 0; s < 3; ++s) sub[s * PV + idx] = 0.f;
+      } else if (masked(pr - r0, pc - c0, hm, wm, d4)) {
+#pragma unroll
+        for (int s = 0; s < 3; ++s) sub[s * PV + idx] = p.ext[s];
+      }
+    }
+  }
+  for (int q = tid; q < h * w; q += NT) {
+    const int i = 1 + q / w, j = 1 + q % w;
+    if (masked(i, j, hm, wm, d4)) {
+      const int idx = (r0 + i) * PW + (c0 + j);
+#pragma unroll
+      for (int s = 0; s < 3; ++s) sub[s * PV + idx] = p.ext[s];
+    }
+  }
+  // Items: band b (rows b*L .. b*L+L-1 of the mask box) x column j.
+  const int nb = (hm + L - 1) / L;
+  const int items = nb * wm;
+  uint32_t act[KMAX], fin[KMAX];
+  int base[KMAX];
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) {
+    const int q = tid + k * NT;
+    act[k] = 0u; fin[k] = 0u; base[k] = 0;
+    if (q < items) {
+      const int b = q / wm, j = q - b * wm;
+      const int i0 = b * L;
+      base[k] = (i0 << 16) | j;
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        const int i = i0 + l;
+        if (i < hm) {
+          if (!masked(i, j, hm, wm, d4)) act[k] |= 1u << l;
+          if (i >= 1 && i <= h && j >= 1 && j <= w) fin[k] |= 1u << l;
+        }
+      }
+    }
+  }
+  const int S = p.substeps;
+  const int total = S > 1 ? S : 1;
+  {
+    uint32_t na = 0, nf = 0;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) { na += __popc(act[k]); nf += __popc(fin[k]); }
+    updates += na * (uint32_t)(total - 1) + nf;
+  }
+  const float R2 = (float)0.70710678118654752440;
+  const float F = (float)(4.0 + (double)(float)(2.0 * 1.41421356237309504880));
+  const int rmax = R1;  // last padded row of the temporary grid
+  __syncthreads();
+  for (int step = 0; step < total; ++step) {
+    const bool final_sub = (step == total - 1);
+    float res[KMAX][L][3];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      const uint32_t m = final_sub ? fin[k] : act[k];
+      if (m == 0u) continue;
+      const int i0 = base[k] >> 16, j = base[k] & 0xffff;
+      const int pc = c0 + j;
+      const int pr0 = r0 + i0;
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const float *P = sub + s * PV;
+        const float kk = p.k[s];
+        const float *up = P + (pr0 - 1) * PW + pc;
+        const float *mid = P + pr0 * PW + pc;
+        float ul = up[-1], uc = up[0], ur = up[1];
+        float ml = mid[-1], mc = mid[0], mr = mid[1];
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+          const int prd = min(pr0 + l + 1, rmax);
+          const float *dn = P + prd * PW + pc;
+          const float dl = dn[-1], dc = dn[0], dr = dn[1];
+          const float vl = ul + dl, vc = uc + dc, vr = ur + dr;
+          const float o = vc + (ml + mr);
+          const float dd = vl + vr;
+          float Lp = fmaf(dd, R2, o);
+          Lp = fmaf(-F, mc, Lp);
+          res[k][l][s] = fmaf(kk, Lp, mc);
+          ul = ml; uc = mc; ur = mr;
+          ml = dl; mc = dc; mr = dr;
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      const uint32_t m = final_sub ? fin[k] : act[k];
+      if (m == 0u) continue;
+      const int i0 = base[k] >> 16, j = base[k] & 0xffff;
+      const int idx0 = (r0 + i0) * PW + (c0 + j);
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        if (m & (1u << l)) {
+#pragma unroll
+          for (int s = 0; s < 3; ++s) sub[s * PV + idx0 + l * PW] = res[k][l][s];
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < PS; ++k) {
+    if (sidx[k] >= 0) {
+#pragma unroll
+      for (int s = 0; s < 3; ++s) sub[s * PV + sidx[k]] = sv[k][s];
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Cell update (cellSimulationKernel, simulation.cu:66-110, with Site::step,
+// site.h:62-82) under deterministic rules:
+//   * every live slot (occupied site, newest list entry) is stepped once;
+//   * the vacant-neighbour count uses the site states before the step;
+//   * a cell ready to divide draws a neighbour; it is a candidate when that
+//     neighbour is vacant after every death of the step;
+//   * at most one division per lattice per step, won by the smallest
+//     arbitration_key of the candidates' list positions;
+//   * the child is appended to the occupied list.
+// Site substrates are reached through `SubRef` (LDS in the persistent kernel,
+// global memory in the per-step kernel).  During the step a dying site keeps
+// its OCCUPIED bits with 0x80 set, so neighbour counts still see it occupied;
+// the marks are cleared before the division is applied.
+// ---------------------------------------------------------------------------
+struct SubLds {
+  float *sub;
+  int PW, PV;
+  __device__ float *at(int s, int r, int c) const { return sub + s * PV + (r + 1) * PW + (c + 1); }
+};
+struct SubGlobal {
+  float *sub;
+  int W, HW;
+  __device__ float *at(int s, int r, int c) const { return sub + s * HW + r * W + c; }
+};
+
+template <int NT, typename SubRef>
+__device__ void cells_step(const DevParams &p, const RngKey &key, uint32_t t, float dose,
+                           const SubRef &sr, uint8_t *st, int H, int W, uint32_t *slots,
+                           int cap, uint64_t *ovf, Shared &sm) {
+  const int tid = threadIdx.x;
+  if (tid == 0) { sm.ready_cnt = 0; sm.winner = kNoWinner; }
+  __syncthreads();
+  const int n = sm.n;
+  uint32_t *S = slots + (int64_t)sm.cur * kSlotFields * cap;
+  for (int i = tid; i < n; i += NT) {
+    const uint32_t w = S[(int64_t)kPacked * cap + i];
+    if (!slot_live(w)) continue;
+    const int x = (int)slot_site(w);
+    const int r = x / W, c = x - r * W;
+    const int vn = ((st[x - W - 1] & 3) == kVacant) + ((st[x - W + 1] & 3) == kVacant) +
+                   ((st[x + W - 1] & 3) == kVacant) + ((st[x + W + 1] & 3) == kVacant) +
+                   ((st[x - 1] & 3) == kVacant) + ((st[x + 1] & 3) == kVacant) +
+                   ((st[x - W] & 3) == kVacant) + ((st[x + W] & 3) == kVacant);
+    CellL cl;
+    cl.mode = slot_mode(w);
+    cl.phase = slot_phase(w);
+    cl.prolif = __uint_as_float(S[(int64_t)kProlif * cap + i]);
+    cl.t_rep = __uint_as_float(S[(int64_t)kTRep * cap + i]);
+    cl.R = __uint_as_float(S[(int64_t)kR * cap + i]);
+    cl.t0 = __uint_as_float(S[(int64_t)kT0 * cap + i]);
+    cl.t4 = __uint_as_float(S[(int64_t)kT4 * cap + i]);
+    const uint32_t ph = cl.phase > 4u ? 4u : cl.phase;
+    cl.tcur = ph == 0u ? cl.t0
+            : ph == 4u ? cl.t4 : __uint_as_float(S[(int64_t)(kT0 + ph) * cap + i]);
+    bool delay_loaded = false;
+    float *ps0 = sr.at(0, r, c), *ps1 = sr.at(1, r, c), *ps2 = sr.at(2, r, c);
+    float cho = *ps0, ox = *ps1, gi = *ps2;
+    const float t_rep0 = cl.t_rep, R0 = cl.R;
+    const float prolif0 = cl.prolif;
+    bool alive = update_state(cl, cho, ox, gi, p, vn);
+    bool ready = false;
+    float delay = 0.f;
+    if (alive) {
+      if (dose > 0.f) {
+        irradiate(cl, dose, p);
+        delay = cl.delay;
+        delay_loaded = true;
+      }
+      cho = cho - p.metab[cl.mode][0];
+      ox = ox - p.metab[cl.mode][1];
+      gi = gi - p.metab[cl.mode][2];
+      const bool changed = progress_clock(cl, p);
+      if (cl.t_rep > 0.f || (cl.R > 0.f && changed)) {
+        cl.t_rep = cl.t_rep + p.hour_step;
+        if (!delay_loaded) {
+          delay = __uint_as_float(S[(int64_t)kDelay * cap + i]);
+          delay_loaded = true;
+        }
+        if (cl.t_rep >= delay) {
+          const float death_prob = 1.f - p.surv_coeff * expf_fixed(p.surv_exp * cl.R);
+          const float u = uniform(key, t, (uint32_t)x, kPurposeRepair);
+          if (u < death_prob) {
+            alive = false;
+          } else {
+            cl.R = 0.f;
+            cl.t_rep = 0.f;
+            delay = 0.f;
+          }
+        }
+      }
+      *ps0 = cho; *ps1 = ox; *ps2 = gi;
+      if (alive) ready = (cl.phase == 4u);
+    }
+    if (!alive) {
+      st[x] = (uint8_t)(kOccupied | 0x80);
+      S[(int64_t)kPacked * cap + i] = w & ~kLiveBit;
+    } else {
+      const uint32_t nw = pack_slot((uint32_t)x, cl.mode, cl.phase, true);
+      if (nw != w) S[(int64_t)kPacked * cap + i] = nw;
+      if (cl.prolif != prolif0) S[(int64_t)kProlif * cap + i] = __float_as_uint(cl.prolif);
+      if (cl.t_rep != t_rep0) S[(int64_t)kTRep * cap + i] = __float_as_uint(cl.t_rep);
+      if (cl.R != R0) S[(int64_t)kR * cap + i] = __float_as_uint(cl.R);
+      if (delay_loaded) S[(int64_t)kDelay * cap + i] = __float_as_uint(delay);
+      if (ready) {
+        const float u = uniform(key, t, (uint32_t)x, kPurposeNeighbour);
+        int dr, dc;
+        choose_neighbour(u, dr, dc);
+        const uint32_t child = (uint32_t)((r + dr) * W + (c + dc));
+        const uint64_t ent = (uint64_t)(uint32_t)i | ((uint64_t)child << 16) |
+                             ((uint64_t)__float_as_uint(cl.R) << 32);
+        const int e = atomicAdd(&sm.ready_cnt, 1);
+        if (e < kReadyLds) sm.ready[e] = ent; else ovf[e - kReadyLds] = ent;
+      }
+    }
+  }
+  __syncthreads();
+  // Candidates: the child must be vacant after every death of this step.
+  const int cnt = sm.ready_cnt;
+  uint32_t best = kNoWinner;
+  uint64_t best_ent = 0;
+  for (int e = tid; e < cnt; e += NT) {
+    const uint64_t ent = e < kReadyLds ? sm.ready[e] : ovf[e - kReadyLds];
+    const uint32_t slot = (uint32_t)(ent & 0xffffu);
+    const uint32_t child = (uint32_t)((ent >> 16) & 0xffffu);
+    const uint8_t v = st[child];
+    if ((v & 3) == kVacant || (v & 0x80)) {
+      const uint32_t kk = arbitration_key(slot);
+      if (kk < best) { best = kk; best_ent = ent; }
+    }
+  }
+  if (best != kNoWinner) atomicMin(&sm.winner, best);
+  // Clear the death marks (a mark or VACANT both read as vacant above).
+  uint32_t *st32 = reinterpret_cast<uint32_t *>(st);
+  const int nw32 = (H * W + 3) >> 2;
+  for (int k = tid; k < nw32; k += NT) {
+    const uint32_t v = st32[k];
+    if (v & 0x80808080u) {
+      uint32_t out = v;
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if (v & (0x80u << (8 * b))) out &= ~(0xffu << (8 * b));
+      st32[k] = out;
+    }
+  }
+  __syncthreads();
+  if (best != kNoWinner && best == sm.winner) {
+    // divideCell / createCell (cell-division.h:10-30), simulation.cu:99-105.
+    const uint32_t slot = (uint32_t)(best_ent & 0xffffu);
+    const uint32_t child = (uint32_t)((best_ent >> 16) & 0xffffu);
+    const float R = __uint_as_float((uint32_t)(best_ent >> 32));
+    const uint32_t pw = S[(int64_t)kPacked * cap + slot];
+    S[(int64_t)kPacked * cap + slot] = pack_slot(slot_site(pw), slot_mode(pw), 0u, true);
+    S[(int64_t)kProlif * cap + slot] = 0u;
+    float z[5];
+    normals5(key, t, child, z);
+    float acc = 0.f;
+    const int nn = sm.n;
+    if (nn < cap) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const float v = fmaf(p.cycle_sd[k], z[k], p.cycle_mean[k]);
+        acc = (k == 0) ? v : acc + v;
+        S[(int64_t)(kT0 + k) * cap + nn] = __float_as_uint(acc);
+      }
+      S[(int64_t)kTRep * cap + nn] = 0u;
+      S[(int64_t)kR * cap + nn] = __float_as_uint(R);
+      S[(int64_t)kDelay * cap + nn] = __float_as_uint(R > 0.f ? delay_time(R, p) : 0.f);
+      S[(int64_t)kProlif * cap + nn] = 0u;
+      S[(int64_t)kPacked * cap + nn] = pack_slot(child, 0u, 0u, true);
+      st[child] = kOccupied;
+      sm.n = nn + 1;
+      sm.divisions += 1u;
+    } else {
+      sm.status |= 1u;
+    }
+  }
+  __syncthreads();
+}
+
+}  // namespace dev
+}  // namespace emt6ro
+
+#endif  // EMT6RO_AMD_SIM_PHASES_H_

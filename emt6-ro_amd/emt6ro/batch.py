@@ -1,0 +1,89 @@
+"""Batched simulations of uneven lattice sizes and horizons in one launch
+(include/emt6ro_amd.h, emt6ro_batch_*).  The reference runs one lattice size
+and one horizon per Simulation (simulation/simulation.cu:123-137)."""
+import ctypes as C
+
+import numpy as np
+
+from ._native import lib, check, PATH_PERSISTENT, PATH_STEP, PATH_STEP_GRAPH
+from .simulation import TumorState
+
+PATHS = {"persistent": PATH_PERSISTENT, "step": PATH_STEP, "graph": PATH_STEP_GRAPH}
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class Batch:
+    """Simulation i runs tumors[tumor_index[i]] under doses[prot_index[i]]
+    (a row of slot doses; a dose fires at steps s < prot_length with
+    s % prot_res == 0, protocol.h:9-14) until step horizon[i]."""
+
+    def __init__(self, params, tumors, tumor_index, prot_index, horizon, doses, prot_res,
+                 prot_length, sim_id=None, gpu_id=0, seed=0):
+        self._params, self._tumors = params, list(tumors)
+        ti = np.ascontiguousarray(tumor_index, np.int32)
+        pi = np.ascontiguousarray(prot_index, np.int32)
+        hz = np.ascontiguousarray(horizon, np.uint32)
+        n = len(ti)
+        sid = np.ascontiguousarray(np.arange(n) if sim_id is None else sim_id, np.uint32)
+        d = np.ascontiguousarray(doses, np.float32)
+        if d.ndim != 2:
+            raise ValueError("doses must be (n_protocols, n_slots)")
+        arr = (C.c_void_p * len(self._tumors))(*[t._h.value for t in self._tumors])
+        h = C.c_void_p()
+        check(lib.emt6ro_batch_create(params._h, arr, len(self._tumors), n,
+                                      _p(ti, C.c_int32), _p(pi, C.c_int32), _p(hz, C.c_uint32),
+                                      _p(sid, C.c_uint32), d.shape[0], d.shape[1],
+                                      _p(d, C.c_float), int(prot_res), int(prot_length),
+                                      int(gpu_id), int(seed), C.byref(h)))
+        self._h = h
+        self.n = n
+
+    def run(self, nsteps, path="persistent", variant=-1, steps_per_launch=0):
+        """Advance every simulation nsteps (capped by its horizon); returns the
+        device time of the run in ms (HIP events on the batch's stream)."""
+        check(lib.emt6ro_batch_run(self._h, int(nsteps), PATHS[path], int(variant),
+                                   int(steps_per_launch)))
+        return self.last_run()[0]
+
+    def last_run(self):
+        ms, launches = C.c_double(), C.c_int64()
+        check(lib.emt6ro_batch_last_run(self._h, C.byref(ms), C.byref(launches)))
+        return ms.value, launches.value
+
+    def results(self):
+        out = np.zeros(self.n, np.uint32)
+        check(lib.emt6ro_batch_results(self._h, _p(out, C.c_uint32)))
+        return out
+
+    def counters(self):
+        out = np.zeros((self.n, 5), np.uint64)
+        check(lib.emt6ro_batch_counters(self._h, _p(out, C.c_uint64)))
+        return {"step": out[:, 0], "n_list": out[:, 1], "divisions": out[:, 2],
+                "status": out[:, 3], "site_substeps": out[:, 4]}
+
+    def state(self, i):
+        h = C.c_void_p()
+        check(lib.emt6ro_batch_state(self._h, int(i), C.byref(h)))
+        return TumorState(h.value)
+
+    def reset(self, epoch=0):
+        check(lib.emt6ro_batch_reset(self._h, int(epoch)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib.emt6ro_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+
+def variants():
+    """Compiled kernel shapes: dicts of id, nt, l, kmax, ps."""
+    n = lib.emt6ro_variant_info(-1, None)
+    out = []
+    for i in range(n):
+        info = (C.c_int32 * 5)()
+        check(lib.emt6ro_variant_info(i, info))
+        out.append(dict(zip(("id", "nt", "l", "kmax", "ps"), list(info))))
+    return out

@@ -1,0 +1,184 @@
+/*
+ * emt6ro_amd -- C ABI of the MI355X EMT6/Ro simulator (libemt6ro_amd.so).
+ *
+ * Plain C types only; every handle is opaque and every output buffer is
+ * allocated by the caller.  Functions return 0 on success and a negative code
+ * on failure; emt6ro_last_error() then gives the message of the calling
+ * thread's last failure.  Where the reference raises an error through
+ * ENFORCE (common/error.h:25-29) the message has the same text:
+ * "Check failed: <condition>; <message>".
+ *
+ * The `python/emt6ro` package of the reference binds the same operations
+ * through pybind11; each declaration cites the reference function it stands
+ * in for.
+ */
+#ifndef EMT6RO_AMD_H_
+#define EMT6RO_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMT6RO_OK 0
+#define EMT6RO_ERR_ARG -1     /* invalid argument or state */
+#define EMT6RO_ERR_IO -2      /* file could not be read or parsed */
+#define EMT6RO_ERR_DEVICE -3  /* HIP runtime error */
+
+typedef struct emt6ro_params emt6ro_params;
+typedef struct emt6ro_state emt6ro_state;
+typedef struct emt6ro_experiment emt6ro_experiment;
+typedef struct emt6ro_batch emt6ro_batch;
+
+/* Message of this thread's last failed call ("" if none). */
+const char *emt6ro_last_error(void);
+/* Library version string. */
+const char *emt6ro_version(void);
+
+/* ---- Parameters (parameters/parameters.h:10-72) ------------------------- */
+
+/* Number of values in the flat parameter vector:
+ *   [0] lattice height  [1] lattice width  [2] time_step
+ *   [3..5] diffusion coeffs cho, ox, gi  [6] diffusion time_step
+ *   [7..18] metabolism: aerobic prolif., anaerobic prolif., aerobic quiesc.,
+ *           anaerobic quiesc., each cho, ox, gi
+ *   [19..21] external levels cho, ox, gi
+ *   [22] death_gi  [23] quiescence_gi  [24] death_gi_production
+ *   [25..34] cycle times g1, s, g2, m, d, each mean, stddev
+ *   [35] delay coeff  [36] delay exp_coeff  [37] survival coeff
+ *   [38] survival exp_coeff  [39] repair_half_time */
+#define EMT6RO_NUM_PARAMS 40
+
+/* Parameters::loadFromJSONFile (parameters/parameters.cc:66-80);
+ * Python: load_parameters (python/emt6ro/simulation/backend.cc:217). */
+int emt6ro_params_load(const char *path, emt6ro_params **out);
+int emt6ro_params_from_values(const float *values, int n, emt6ro_params **out);
+int emt6ro_params_values(const emt6ro_params *p, float *values, int n);
+void emt6ro_params_free(emt6ro_params *p);
+
+/* ---- Tumour state (state/state.h:10-12; HostGrid<Site>, common/grid.h) -- */
+
+/* Per-site fields of a tumour state; each is an H x W row-major array of the
+ * bordered lattice (H = interior height + 2).  TumorState accessors in
+ * python/emt6ro/simulation/backend.cc:138-179,198-205. */
+enum emt6ro_field {
+  EMT6RO_FIELD_CHO = 0,            /* float */
+  EMT6RO_FIELD_OX = 1,             /* float */
+  EMT6RO_FIELD_GI = 2,             /* float */
+  EMT6RO_FIELD_OCCUPANCY = 3,      /* uint8: 0 vacant, 1 occupied, 2 mocked */
+  EMT6RO_FIELD_TIME_IN_REPAIR = 4, /* float */
+  EMT6RO_FIELD_IRRADIATION = 5,    /* float */
+  EMT6RO_FIELD_REPAIR_DELAY = 6,   /* float */
+  EMT6RO_FIELD_PROLIF_TIME = 7,    /* float */
+  EMT6RO_FIELD_CYCLE_G1 = 8,       /* float, cumulative phase ends ... */
+  EMT6RO_FIELD_CYCLE_S = 9,
+  EMT6RO_FIELD_CYCLE_G2 = 10,
+  EMT6RO_FIELD_CYCLE_M = 11,
+  EMT6RO_FIELD_CYCLE_D = 12,
+  EMT6RO_FIELD_MODE = 13,          /* uint8 metabolic mode */
+  EMT6RO_FIELD_PHASE = 14,         /* uint8 cycle phase */
+  EMT6RO_NUM_FIELDS = 15
+};
+
+/* loadFromFile (state/state.cc:9-53); Python: load_state (backend.cc:219). */
+int emt6ro_state_load(const char *path, const emt6ro_params *p, emt6ro_state **out);
+/* saveToFile (state/state.cc:55-82). */
+int emt6ro_state_save(const emt6ro_state *s, const char *path);
+/* An all-vacant state of bordered dims H x W with the border MOCKED and every
+ * site at the external levels of `p`; fields are then set with
+ * emt6ro_state_set_field. */
+int emt6ro_state_create(int32_t H, int32_t W, const emt6ro_params *p, emt6ro_state **out);
+int emt6ro_state_dims(const emt6ro_state *s, int32_t *H, int32_t *W);
+int emt6ro_state_get_field(const emt6ro_state *s, int field, void *dst);
+int emt6ro_state_set_field(emt6ro_state *s, int field, const void *src);
+void emt6ro_state_free(emt6ro_state *s);
+
+/* ---- Experiment (python/emt6ro/simulation/backend.cc:16-136) ------------ */
+
+/* _Experiment(params, tumors, tests_num, protocols_num, sim_steps,
+ * prot_resolution, device_id) (backend.cc:38-76).  Simulation b = p*T*R +
+ * t*R + r runs tumour t under protocol p (replicate r).  `seed` keys the
+ * counter-based random stream (the reference draws it from
+ * std::random_device; pass any value, the same seed gives the same run). */
+int emt6ro_experiment_create(const emt6ro_params *p, const emt6ro_state *const *tumors,
+                             int32_t num_tumors, int32_t runs, int32_t num_protocols,
+                             uint32_t sim_steps, uint32_t resolution, int32_t device,
+                             uint64_t seed, emt6ro_experiment **out);
+/* addIrradiations (backend.cc:91-101): protocol i's (step, dose) pairs are
+ * steps[offsets[i] .. offsets[i+1]) and doses[...]; doses accumulate into
+ * slot step / resolution.  `num_protocols` must equal the experiment's. */
+int emt6ro_experiment_add_irradiations(emt6ro_experiment *e, int32_t num_protocols,
+                                       const int32_t *offsets, const int32_t *steps,
+                                       const float *doses);
+/* run (backend.cc:103-113): starts nsteps steps and returns at once. */
+int emt6ro_experiment_run(emt6ro_experiment *e, uint32_t nsteps);
+/* results (backend.cc:115-121): waits for the run; dst holds P*T*R counts. */
+int emt6ro_experiment_results(emt6ro_experiment *e, uint32_t *dst);
+/* reset (backend.cc:84-89): initial states back, doses zeroed. */
+int emt6ro_experiment_reset(emt6ro_experiment *e);
+/* state (backend.cc:123-134): copy of simulation `sample`. */
+int emt6ro_experiment_state(emt6ro_experiment *e, int32_t sample, emt6ro_state **out);
+int emt6ro_experiment_batch_size(const emt6ro_experiment *e);
+/* Kernel path used by later runs: 0 persistent (default), 1 per-step. */
+int emt6ro_experiment_set_path(emt6ro_experiment *e, int32_t path);
+void emt6ro_experiment_destroy(emt6ro_experiment *e);
+
+/* ---- Batch: uneven simulations in one launch (no reference counterpart;
+ * the reference runs one lattice size and one horizon per Simulation,
+ * simulation.cu:123-137) ------------------------------------------------- */
+
+/* Simulation i runs tumours[tumor_index[i]] under protocol
+ * prot_index[i] (a row of n_prot x n_slots doses in `prot_doses`) up to
+ * step horizon[i]; its random stream is sim_id[i].  Doses fire at steps
+ * s < prot_length with s % prot_res == 0 (protocol.h:9-14). */
+int emt6ro_batch_create(const emt6ro_params *p, const emt6ro_state *const *tumors,
+                        int32_t num_tumors, int32_t num_sims, const int32_t *tumor_index,
+                        const int32_t *prot_index, const uint32_t *horizon,
+                        const uint32_t *sim_id, int32_t n_prot, int32_t n_slots,
+                        const float *prot_doses, uint32_t prot_res, uint32_t prot_length,
+                        int32_t device, uint64_t seed, emt6ro_batch **out);
+/* Runs every simulation `nsteps` further (capped by its horizon) and waits.
+ * path: 0 persistent kernel, 1 per-step kernels, 2 per-step kernels
+ * replayed from a captured HIP graph.  variant: kernel shape from
+ * emt6ro_variant_info, or -1 for the tuned / default choice.
+ * steps_per_launch: persistent kernel steps per launch (0: all at once). */
+int emt6ro_batch_run(emt6ro_batch *b, uint32_t nsteps, int32_t path, int32_t variant,
+                     uint32_t steps_per_launch);
+/* Living-cell counts (num_sims values). */
+int emt6ro_batch_results(emt6ro_batch *b, uint32_t *dst);
+/* Per-simulation counters: step, list length, divisions, status and
+ * diffusion site updates, 5 values each (uint64). */
+int emt6ro_batch_counters(emt6ro_batch *b, uint64_t *dst);
+int emt6ro_batch_state(emt6ro_batch *b, int32_t sim, emt6ro_state **out);
+/* Restores the initial states; `epoch` selects a fresh random stream. */
+int emt6ro_batch_reset(emt6ro_batch *b, uint32_t epoch);
+/* Milliseconds of the last run on the device (HIP events on its stream),
+ * and the number of kernel launches it made. */
+int emt6ro_batch_last_run(const emt6ro_batch *b, double *ms, int64_t *launches);
+void emt6ro_batch_destroy(emt6ro_batch *b);
+
+/* ---- Kernel variants and tuning ----------------------------------------- */
+
+/* info[0..4] = id, workgroup size, diffusion band rows, items per thread,
+ * ring slots per thread.  Returns the number of variants if idx < 0. */
+int emt6ro_variant_info(int32_t idx, int32_t *info);
+/* Workgroups per CU the persistent kernel of `variant` reaches for lattices
+ * up to hmax x wmax (0 if it does not fit). */
+int emt6ro_variant_occupancy(int32_t variant, int32_t hmax, int32_t wmax);
+
+/* ---- Instrumentation (common/instr.h:33-58; backend.cc:221-231) -------- */
+
+/* The reference compiles its device counters only with EMT6RO_INSTRUMENT
+ * and reads zeros otherwise; this build keeps per-simulation division
+ * counts (emt6ro_batch_counters) and reports the global counters as zero. */
+int emt6ro_instr_read(uint64_t counters[4], double *sum_death_prob);
+int emt6ro_instr_reset(void);
+int emt6ro_instr_histograms(uint64_t *dst, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMT6RO_AMD_H_ */

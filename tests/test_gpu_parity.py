@@ -1,0 +1,198 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
+
+Integer bookkeeping (site states, list lengths, phases, modes) and every
+float (substrates, cell clocks) must match the oracle bit for bit: both follow
+the same operation order with the same counter-based random numbers.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+FLOAT_FIELDS = ("time_in_repair", "irradiation", "repair_delay", "proliferation_time",
+                "cycle_g1", "cycle_s", "cycle_g2", "cycle_m", "cycle_d")
+ORACLE_CELL = {"time_in_repair": ("t_rep", None), "irradiation": ("R", None),
+               "repair_delay": ("delay", None), "proliferation_time": ("prolif", None),
+               "cycle_g1": ("times", 0), "cycle_s": ("times", 1), "cycle_g2": ("times", 2),
+               "cycle_m": ("times", 3), "cycle_d": ("times", 4)}
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def sim():
+    import emt6ro.simulation as sim
+    return sim
+
+
+@pytest.fixture(scope="module")
+def params(sim):
+    return sim.load_parameters(golden("default-parameters.json"))
+
+
+def oracle_state_from_arrays(arrays, values):
+    H, W = arrays["occupancy"].shape
+    s = O.empty_state(values, H, W)
+    s["sub"][0], s["sub"][1], s["sub"][2] = arrays["cho"], arrays["ox"], arrays["gi"]
+    s["st"][:] = arrays["occupancy"]
+    cells = s["cells"].reshape(H, W)
+    occ = arrays["occupancy"] == 1
+    for name, (f, k) in ORACLE_CELL.items():
+        if k is None:
+            cells[f][occ] = arrays[name][occ]
+        else:
+            cells[f][occ, k] = arrays[name][occ]
+    cells["mode"][occ] = arrays["mode"][occ]
+    cells["phase"][occ] = arrays["phase"][occ]
+    return s
+
+
+def inputs(sim, params, n_synth=2):
+    """Library tumours 1-3 and synthetic disks, each as (product state, oracle state)."""
+    from emt6ro.synthetic import disk_state
+    v = params.values()
+    out = []
+    for t in (1, 2, 3):
+        path = golden("tumor-lib", f"tumor-{t}.txt")
+        out.append((sim.load_state(path, params), O.read_state_text(path, v)))
+    for k in range(n_synth):
+        st, arrays = disk_state(params, 53, 53, radius=8 + 3 * k, seed=100 + k)
+        out.append((st, oracle_state_from_arrays(arrays, v)))
+    return out
+
+
+def compare_states(gpu_state, orc, b, where):
+    H, W = gpu_state.shape
+    occ_g = gpu_state.occupancy()
+    occ_o = orc.st[b]
+    assert np.array_equal(occ_g, occ_o), f"{where}: site states differ"
+    for k, name in enumerate(("cho", "ox", "gi")):
+        g = gpu_state.field(name)
+        o = orc.sub[b, k]
+        if not np.array_equal(g.view(np.uint32), o.view(np.uint32)):
+            bad = np.argwhere(g.view(np.uint32) != o.view(np.uint32))
+            r, c = bad[0]
+            raise AssertionError(f"{where}: {name} differs at {len(bad)} sites, first "
+                                 f"({r},{c}) gpu={g[r, c]!r} oracle={o[r, c]!r}")
+    occ = occ_o == 1
+    cells = orc.cells[b].reshape(H, W)
+    for name, (f, k) in ORACLE_CELL.items():
+        g = gpu_state.field(name)[occ]
+        o = cells[f][occ] if k is None else cells[f][occ, k]
+        assert np.array_equal(g.view(np.uint32), o.astype(np.float32).view(np.uint32)), \
+            f"{where}: {name} differs"
+    assert np.array_equal(gpu_state.mode()[occ], cells["mode"][occ]), f"{where}: mode differs"
+    assert np.array_equal(gpu_state.phase()[occ], cells["phase"][occ]), f"{where}: phase differs"
+
+
+def build_pair(sim, params, doses, reps, seed):
+    from emt6ro.batch import Batch
+    ins = inputs(sim, params)
+    T = len(ins)
+    n = T * reps
+    tumor_index = np.repeat(np.arange(T), reps)
+    prot_index = np.arange(n) % doses.shape[0]
+    horizon = np.full(n, 2 ** 32 - 1, np.uint32)
+    b = Batch(params, [s for s, _ in ins], tumor_index, prot_index, horizon, doses, 300, 144000,
+              sim_id=np.arange(n), seed=seed)
+    orc = O.OracleBatch(params.values(), [ins[t][1] for t in tumor_index], doses, prot_index,
+                        np.arange(n), 300, 144000)
+    return b, orc
+
+
+def dose_table():
+    d = np.zeros((3, 480), np.float32)
+    d[0, 0] = 2.0          # dose at step 0
+    d[1, 1] = 1.25         # step 300
+    d[1, 3] = 3.0          # step 900
+    return d
+
+
+@pytest.mark.parametrize("path", ["persistent", "step", "graph"])
+def test_paths_match_oracle_bitexact(sim, params, path):
+    nsteps = {"persistent": 3000, "step": 400, "graph": 300}[path]
+    log(f"[parity] path={path} nsteps={nsteps}")
+    b, orc = build_pair(sim, params, dose_table(), reps=2, seed=12345)
+    b.run(nsteps, path=path)
+    orc.run(nsteps, seed=12345, nthreads=8)
+    got = b.results().astype(np.int64)
+    assert np.array_equal(got, orc.counts()), (got, orc.counts())
+    cnt = b.counters()
+    assert (cnt["step"] == nsteps).all()
+    assert (cnt["status"] == 0).all()
+    assert np.array_equal(cnt["n_list"].astype(np.int64), orc.nlist.astype(np.int64))
+    for i in range(b.n):
+        compare_states(b.state(i), orc, i, f"{path} sim {i}")
+
+
+def test_long_run_with_repair_and_division_matches_oracle(sim, params):
+    """Past the repair delay (> 3.3 h) so repair deaths and divisions occur."""
+    log("[parity] long run 6000 steps")
+    b, orc = build_pair(sim, params, dose_table(), reps=2, seed=777)
+    b.run(6000)
+    orc.run(6000, seed=777, nthreads=8)
+    assert np.array_equal(b.results().astype(np.int64), orc.counts())
+    cnt = b.counters()
+    assert cnt["divisions"].sum() > 0
+    for i in range(b.n):
+        compare_states(b.state(i), orc, i, f"long sim {i}")
+
+
+def test_split_runs_and_launch_chunks_equal_one_run(sim, params):
+    log("[parity] split runs")
+    d = dose_table()
+    b1, _ = build_pair(sim, params, d, reps=1, seed=5)
+    b2, _ = build_pair(sim, params, d, reps=1, seed=5)
+    b3, _ = build_pair(sim, params, d, reps=1, seed=5)
+    b1.run(1000)
+    b2.run(313)
+    b2.run(687, path="step")
+    b3.run(1000, steps_per_launch=97)
+    r1 = b1.results()
+    assert np.array_equal(r1, b2.results())
+    assert np.array_equal(r1, b3.results())
+    for i in range(b1.n):
+        s1, s2, s3 = b1.state(i), b2.state(i), b3.state(i)
+        for name in ("cho", "ox", "gi", "occupancy", "proliferation_time"):
+            assert np.array_equal(s1.field(name), s2.field(name))
+            assert np.array_equal(s1.field(name), s3.field(name))
+
+
+def test_every_variant_matches(sim, params):
+    from emt6ro.batch import variants
+    log("[parity] variants")
+    d = dose_table()
+    ref = None
+    for v in variants():
+        b, _ = build_pair(sim, params, d, reps=1, seed=3)
+        b.run(500, variant=v["id"])
+        st = [b.state(i) for i in range(b.n)]
+        sig = [np.concatenate([s.field(n).view(np.uint32).ravel() if s.field(n).dtype != np.uint8
+                               else s.field(n).astype(np.uint32).ravel()
+                               for n in ("cho", "ox", "gi", "occupancy")]) for s in st]
+        if ref is None:
+            ref = sig
+        else:
+            for a, bb in zip(ref, sig):
+                assert np.array_equal(a, bb), f"variant {v} differs"
+
+
+def test_reset_restores_initial_state(sim, params):
+    b, _ = build_pair(sim, params, dose_table(), reps=1, seed=1)
+    r0 = b.results()
+    b.run(700)
+    b.reset(epoch=0)
+    assert np.array_equal(b.results(), r0)
+    b.run(700)
+    first = b.results()
+    b.reset(epoch=0)
+    b.run(700)
+    assert np.array_equal(b.results(), first)
