@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Benchmark: simulated tumour-hours per second on one MI355X.
+
+Workload (batched uniform): 10 seeded synthetic 53x53 tumours (occupied disks
+of radius 8-14) x 100 random dose protocols x 20 replicates = 20,000
+simulations, 144 simulated hours (86,400 steps of 6 s), FP32 substrates.
+
+One bench step advances every simulation by one simulated hour (600 model
+steps) in one launch of the persistent kernel; --steps 144 (the default) is
+the whole 144 h workload.  Warm-up steps run on the same batch, which is then
+reset to its initial states (untimed) before the timed steps.
+
+Also reported, on bounded samples:
+  * the per-step-launch path (the reference's structure: one launch per
+    phase per model step) on the same 20,000 simulations for 1 h;
+  * the CPU oracle on this host's cores for 16 of the simulations over the
+    full 144 h (cpu_baseline).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "emt6-ro_amd"))
+
+METRIC = "simulated tumour-hours/sec on 1 MI355X; GA evals/sec surrogate vs plain"
+FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 peak
+FLOP_PER_SITE_SUBSTEP = 36    # 3 substrates x (6 adds + 3 fma x 2)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=144, help="timed simulated hours")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed simulated hours")
+    ap.add_argument("--variant", type=int, default=-1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the per-step path sample")
+    ap.add_argument("--cpu-sims", type=int, default=16)
+    ap.add_argument("--json-out", default=None)
+    return ap.parse_args()
+
+
+def cpu_baseline(params, wl, n_sims, hours):
+    """The CPU oracle (test infrastructure) on this host's cores, on the first
+    n_sims simulations of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    v = params.values()
+    states = []
+    cache = {}
+    for i in range(n_sims):
+        t = int(wl["tumor_index"][i])
+        if t not in cache:
+            st = wl["tumors"][t]
+            H, W = st.shape
+            s = O.empty_state(v, H, W)
+            s["sub"][0], s["sub"][1], s["sub"][2] = st.cho(), st.ox(), st.gi()
+            s["st"][:] = st.occupancy()
+            cells = s["cells"].reshape(H, W)
+            occ = st.occupancy() == 1
+            cells["t_rep"][occ] = st.field("time_in_repair")[occ]
+            cells["R"][occ] = st.field("irradiation")[occ]
+            cells["delay"][occ] = st.field("repair_delay")[occ]
+            cells["prolif"][occ] = st.field("proliferation_time")[occ]
+            for k, n in enumerate(("cycle_g1", "cycle_s", "cycle_g2", "cycle_m", "cycle_d")):
+                cells["times"][occ, k] = st.field(n)[occ]
+            cells["mode"][occ] = st.mode()[occ]
+            cells["phase"][occ] = st.phase()[occ]
+            cache[t] = s
+        states.append(cache[t])
+    cores = max(1, min(16, os.cpu_count() or 1, n_sims))
+    orc = O.OracleBatch(v, states, wl["doses"], wl["prot_index"][:n_sims],
+                        np.arange(n_sims), 300, wl["prot_length"])
+    t0 = time.perf_counter()
+    orc.run(hours * 600, seed=0, nthreads=cores)
+    dt = time.perf_counter() - t0
+    return {"value": n_sims * hours / dt, "unit": "tumour-hours/s", "cores": cores,
+            "kind": "port", "seconds": dt,
+            "sample": f"{n_sims} of the 20000 simulations (first in batch order) over the "
+                      f"full {hours} h, CPU oracle (C, FP32, OpenMP over simulations)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the simulations are "
+              "independent, so N GPUs would each run 1/N of the batch.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import emt6ro.simulation as sim
+    from emt6ro.batch import Batch, variants
+    from emt6ro.workloads import batched_uniform
+
+    params = sim.load_parameters(os.path.join(ROOT, "tests", "golden", "default-parameters.json"))
+    wl = batched_uniform(params, seed=0)
+    n = len(wl["tumor_index"])
+    b = Batch(params, wl["tumors"], wl["tumor_index"], wl["prot_index"], wl["horizon"],
+              wl["doses"], 300, wl["prot_length"], sim_id=np.arange(n), seed=0)
+    hour = 600
+    for _ in range(args.warmup):
+        b.run(hour, path="persistent", variant=args.variant)
+    b.reset(0)
+    # Timed region: every simulation advances args.steps hours.
+    event_ms = 0.0
+    before = b.counters()["site_substeps"].astype(np.float64).sum()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        event_ms += b.run(hour, path="persistent", variant=args.variant)
+    wall = time.perf_counter() - t0
+    after = b.counters()["site_substeps"].astype(np.float64).sum()
+    living = b.results()
+    value = n * args.steps / wall
+    flops = (after - before) * FLOP_PER_SITE_SUBSTEP
+    launch_s = event_ms / 1e3 / max(args.steps, 1)
+    achieved = flops / max(args.steps, 1) / launch_s / 1e12
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "tumour-hours/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / max(args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched uniform: 10 synthetic 53x53 tumours (disk radius 8-14) x 100 "
+                        "random protocols (<=10 doses of 0.25-1.25 Gy, <=10 Gy, 0.5 h grid, "
+                        "first 72 h) x 20 replicates = 20000 simulations, 144 h horizon",
+            "sims": n, "lattice": "53x53", "hours_timed": args.steps, "path": "persistent",
+            "variant": args.variant, "mean_living": float(living.mean()),
+        },
+        "roofline": {
+            "bound": "mfma", "pipe": "valu_fp32",
+            "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+            "kernel": "persistent_kernel", "launch_ms": launch_s * 1e3,
+            "site_substeps_per_launch": (after - before) / max(args.steps, 1),
+            "flop_per_site_substep": FLOP_PER_SITE_SUBSTEP,
+        },
+    }
+    if not args.no_plain:
+        b.reset(0)
+        t0 = time.perf_counter()
+        b.run(hour, path="graph", variant=args.variant)
+        plain = n / (time.perf_counter() - t0)
+        out["plain_per_step_launch"] = {
+            "value": plain, "unit": "tumour-hours/s",
+            "sample": "the same 20000 simulations, first hour (600 steps x 3 launches, "
+                      "replayed from a captured HIP graph)"}
+        b.reset(0)
+        t0 = time.perf_counter()
+        b.run(hour, path="persistent", variant=args.variant)
+        pers1 = n / (time.perf_counter() - t0)
+        out["persistent_first_hour"] = {"value": pers1, "unit": "tumour-hours/s"}
+        out["speedup_vs_plain_first_hour"] = pers1 / plain
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(params, wl, args.cpu_sims, 144)
+    line = json.dumps(out)
+    print(line, flush=True)
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            f.write(line + "\n")
+
+
+if __name__ == "__main__":
+    main()

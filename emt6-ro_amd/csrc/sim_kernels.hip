@@ -161,7 +161,8 @@ __global__ __launch_bounds__(NT) void diffuse_kernel(RunArgs a) {
 #pragma unroll
     for (int s = 0; s < 3; ++s) sub[s * PV + pr * PW + pc] = in ? g[s * HW + x] : 0.f;
   }
-  (void)sm;
+  // diffuse() rewrites ring and masked sites in place: every load must land first.
+  __syncthreads();
   uint32_t updates = 0u;
   diffuse<NT, L, KMAX, PS>(a.p, sub, PW, PV, roi, updates);
   float *gw = a.sub_pool + d.sub_off;
