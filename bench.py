@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the per-step path sample")
     ap.add_argument("--cpu-sims", type=int, default=16)
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner")
+    ap.add_argument("--tune-steps", type=int, default=600)
     ap.add_argument("--json-out", default=None)
     return ap.parse_args()
 
@@ -101,6 +103,11 @@ def main():
     b = Batch(params, wl["tumors"], wl["tumor_index"], wl["prot_index"], wl["horizon"],
               wl["doses"], 300, wl["prot_length"], sim_id=np.arange(n), seed=0)
     hour = 600
+    tuned = None
+    if args.variant < 0 and not args.no_tune:
+        from emt6ro.tune import autotune
+        tuned = autotune(b, {56: (53, 53)}, steps=args.tune_steps, use_cache=False,
+                         log=lambda m: print(m, file=sys.stderr, flush=True))
     for _ in range(args.warmup):
         b.run(hour, path="persistent", variant=args.variant)
     b.reset(0)
@@ -136,6 +143,7 @@ def main():
                         "first 72 h) x 20 replicates = 20000 simulations, 144 h horizon",
             "sims": n, "lattice": "53x53", "hours_timed": args.steps, "path": "persistent",
             "variant": args.variant, "mean_living": float(living.mean()),
+            "tuned": tuned,
         },
         "roofline": {
             "bound": "mfma", "pipe": "valu_fp32",

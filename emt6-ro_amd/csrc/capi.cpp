@@ -391,6 +391,23 @@ int emt6ro_batch_last_run(const emt6ro_batch *b, double *ms, int64_t *launches) 
 
 void emt6ro_batch_destroy(emt6ro_batch *b) { delete b; }
 
+int emt6ro_batch_set_variant(emt6ro_batch *b, int32_t rs, int32_t variant) {
+  return guarded([&] {
+    need(b != nullptr, "null pointer");
+    int cnt = 0;
+    emt6ro::variant_table(&cnt);
+    need(variant >= -1 && variant < cnt, "variant out of range");
+    b->b->set_preferred_variant(rs, variant);
+  });
+}
+
+int emt6ro_batch_set_doses(emt6ro_batch *b, const float *prot_doses) {
+  return guarded([&] {
+    need(b && prot_doses, "null pointer");
+    b->b->set_protocols(prot_doses);
+  });
+}
+
 int emt6ro_variant_info(int32_t idx, int32_t *info) {
   int cnt = 0;
   const emt6ro::VariantInfo *v = emt6ro::variant_table(&cnt);
@@ -402,6 +419,7 @@ int emt6ro_variant_info(int32_t idx, int32_t *info) {
     info[2] = v[idx].l;
     info[3] = v[idx].kmax;
     info[4] = v[idx].ps;
+    info[5] = v[idx].rs;
   });
 }
 
@@ -409,7 +427,8 @@ int emt6ro_variant_occupancy(int32_t variant, int32_t hmax, int32_t wmax) {
   int cnt = 0;
   const emt6ro::VariantInfo *v = emt6ro::variant_table(&cnt);
   if (variant < 0 || variant >= cnt || !emt6ro::variant_fits(v[variant], hmax, wmax)) return 0;
-  return emt6ro::persistent_occupancy(variant, emt6ro::persistent_lds_bytes(hmax, wmax));
+  return emt6ro::persistent_occupancy(variant,
+                                      emt6ro::persistent_lds_bytes(v[variant], hmax, wmax));
 }
 
 int emt6ro_instr_read(uint64_t counters[4], double *sum_death_prob) {

@@ -47,12 +47,24 @@ class Batch {
   TumorState state(int sim);
   double last_ms() const { return last_ms_; }
   int64_t last_launches() const { return last_launches_; }
-  int choose_variant() const;
+  // Kernel variant for lattices up to hmax x wmax: the fitting variant
+  // with the smallest LDS row stride, earliest in the table.
+  static int choose_variant(int hmax, int wmax);
+  // Preferred variant per LDS row stride class (-1: choose_variant).
+  void set_preferred_variant(int rs, int variant);
+  int class_count() const { return (int)classes_.size(); }
 
  private:
   void sync();
   void ensure_scratch(int wgs);
   dev::RunArgs args(uint32_t nsteps) const;
+  // Simulations grouped by the LDS row stride their lattice needs; each
+  // group is one persistent launch, its simulations longest-first.
+  struct Class {
+    int rs, offset, count, hmax, wmax;
+  };
+  std::vector<Class> classes_;
+  int preferred_[3] = {-1, -1, -1};
 
   HostParams params_;
   DevParams dparams_;

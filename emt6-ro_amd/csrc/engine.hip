@@ -135,14 +135,30 @@ Batch::Batch(const HostParams &p, const std::vector<const TumorState *> &tumors,
     wmax_ = std::max(wmax_, s.W);
     capmax_ = std::max(capmax_, d.cap);
   }
-  // Longest work first: lattice area times horizon.
+  // Size classes by LDS row stride (36 / 56 / 88), longest work first within
+  // each: lattice area times horizon.
+  auto class_of = [](int H, int W) { const int m = std::max(H, W) + 2; return m <= 36 ? 0 : m <= 56 ? 1 : 2; };
+  const int rs_of[3] = {36, 56, 88};
   order_h_.resize(nsims_);
   std::iota(order_h_.begin(), order_h_.end(), 0);
   std::stable_sort(order_h_.begin(), order_h_.end(), [&](int a, int b) {
+    const int ka = class_of(desc_h_[a].H, desc_h_[a].W), kb = class_of(desc_h_[b].H, desc_h_[b].W);
+    if (ka != kb) return ka < kb;
     const double ca = (double)desc_h_[a].H * desc_h_[a].W * desc_h_[a].horizon;
     const double cb = (double)desc_h_[b].H * desc_h_[b].W * desc_h_[b].horizon;
     return ca > cb;
   });
+  for (int i = 0; i < nsims_; ++i) {
+    const SimDesc &d = desc_h_[order_h_[i]];
+    const int k = class_of(d.H, d.W);
+    EMT_ENFORCE(std::max(d.H, d.W) + 2 <= 88, -1, "Lattices are limited to 86 x 86");
+    if (classes_.empty() || classes_.back().rs != rs_of[k])
+      classes_.push_back(Class{rs_of[k], i, 0, 0, 0});
+    Class &c = classes_.back();
+    c.count += 1;
+    c.hmax = std::max(c.hmax, d.H);
+    c.wmax = std::max(c.wmax, d.W);
+  }
   EMT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   EMT_HIP(hipEventCreate(&ev0_));
   EMT_HIP(hipEventCreate(&ev1_));
@@ -250,50 +266,75 @@ dev::RunArgs Batch::args(uint32_t nsteps) const {
   a.scratch_stride = 32 * (int64_t)hmax_;
   a.ovf = ovf_;
   a.ovf_stride = capmax_;
-  a.pv_max = (hmax_ + 2) * (wmax_ + 2);
+  a.pv_max = 0;
   return a;
 }
 
-int Batch::choose_variant() const {
+int Batch::choose_variant(int hmax, int wmax) {
   int cnt = 0;
   const VariantInfo *v = variant_table(&cnt);
+  int best = -1;
   for (int i = 0; i < cnt; ++i)
-    if (variant_fits(v[i], hmax_, wmax_)) return i;
-  return -1;
+    if (variant_fits(v[i], hmax, wmax) && (best < 0 || v[i].rs < v[best].rs)) best = i;
+  return best;
+}
+
+void Batch::set_preferred_variant(int rs, int variant) {
+  const int k = rs <= 36 ? 0 : rs <= 56 ? 1 : 2;
+  preferred_[k] = variant;
 }
 
 void Batch::run(uint32_t nsteps, int path, int variant, uint32_t steps_per_launch) {
   EMT_HIP(hipSetDevice(device_));
   int cnt = 0;
   const VariantInfo *vt = variant_table(&cnt);
-  if (variant < 0) variant = choose_variant();
-  EMT_ENFORCE(variant >= 0 && variant < cnt, -1, "No kernel variant fits lattices of ", hmax_,
-              " x ", wmax_);
-  EMT_ENFORCE(variant_fits(vt[variant], hmax_, wmax_), -1, "Kernel variant ", variant,
-              " does not fit lattices of ", hmax_, " x ", wmax_);
+  EMT_ENFORCE(variant < cnt, -1, "Kernel variant ", variant, " does not exist");
   int64_t launches = 0;
   if (path == kPathPersistent) {
-    const size_t lds = persistent_lds_bytes(hmax_, wmax_);
     int dev_cus = 0;
     EMT_HIP(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, device_));
-    const int occ = persistent_occupancy(variant, lds);
-    EMT_ENFORCE(occ > 0, -1, "Persistent kernel does not fit: ", lds, " bytes of LDS");
-    const int grid = std::min(nsims_, occ * dev_cus);
-    ensure_scratch(grid);
+    struct Plan { int v, grid; size_t lds; };
+    std::vector<Plan> plans;
+    int max_grid = 0;
+    for (const Class &c : classes_) {
+      const int k = c.rs <= 36 ? 0 : c.rs <= 56 ? 1 : 2;
+      int v = variant >= 0 ? variant : preferred_[k];
+      if (v < 0 || !variant_fits(vt[v], c.hmax, c.wmax)) v = choose_variant(c.hmax, c.wmax);
+      EMT_ENFORCE(v >= 0 && variant_fits(vt[v], c.hmax, c.wmax), -1, "Kernel variant ", variant,
+                  " does not fit lattices of ", c.hmax, " x ", c.wmax);
+      const size_t lds = persistent_lds_bytes(vt[v], c.hmax, c.wmax);
+      const int occ = persistent_occupancy(v, lds);
+      EMT_ENFORCE(occ > 0, -1, "Persistent kernel does not fit: ", lds, " bytes of LDS");
+      const int grid = std::min(c.count, occ * dev_cus);
+      plans.push_back(Plan{v, grid, lds});
+      max_grid = std::max(max_grid, grid);
+    }
+    ensure_scratch(max_grid);
     EMT_HIP(hipEventRecord(ev0_, stream_));
     uint32_t done = 0;
     const uint32_t chunk = steps_per_launch ? steps_per_launch : nsteps;
     while (done < nsteps) {
       const uint32_t k = std::min(chunk, nsteps - done);
-      EMT_HIP(launch_fill_i32(queue_, 0, 1, stream_));
-      EMT_HIP(launch_persistent_lds(variant, args(k), grid, lds, stream_));
-      launches += 2;
+      for (size_t ci = 0; ci < classes_.size(); ++ci) {
+        const Class &c = classes_[ci];
+        dev::RunArgs a = args(k);
+        a.order = order_ + c.offset;
+        a.nsims = c.count;
+        a.pv_max = vt[plans[ci].v].rs * (c.hmax + 2);
+        EMT_HIP(launch_fill_i32(queue_, 0, 1, stream_));
+        EMT_HIP(launch_persistent_lds(plans[ci].v, a, plans[ci].grid, plans[ci].lds, stream_));
+        launches += 2;
+      }
       done += k;
     }
     EMT_HIP(hipEventRecord(ev1_, stream_));
   } else if (path == kPathStep || path == kPathStepGraph) {
+    if (variant < 0) variant = choose_variant(hmax_, wmax_);
+    EMT_ENFORCE(variant >= 0 && variant_fits(vt[variant], hmax_, wmax_), -1, "Kernel variant ",
+                variant, " does not fit lattices of ", hmax_, " x ", wmax_);
     ensure_scratch(nsims_);
-    const dev::RunArgs a = args(1);
+    dev::RunArgs a = args(1);
+    a.pv_max = vt[variant].rs * (hmax_ + 2);
     EMT_HIP(hipEventRecord(ev0_, stream_));
     uint32_t done = 0;
     if (path == kPathStepGraph) {

@@ -10,20 +10,22 @@ namespace emt6ro {
 namespace dev { struct RunArgs; }
 
 // A compiled shape of the simulation kernels: workgroup size `nt`, diffusion
-// band height `l`, diffusion items per thread `kmax` and ring-save slots per
-// thread `ps`.  A variant fits a lattice of bordered dims H x W when
-// ceil(H / l) * W <= kmax * nt and 4 (H + W) <= ps * nt.
+// band rows `l`, diffusion items per thread `kmax`, ring-save slots per
+// thread `ps` and LDS row stride `rs`.  A variant fits lattices of bordered
+// dims H x W when H + 2 <= rs, W + 2 <= rs, ceil(H / l) * W <= kmax * nt and
+// 4 (H + W) <= ps * nt.
 struct VariantInfo {
-  int id, nt, l, kmax, ps;
+  int id, nt, l, kmax, ps, rs;
 };
 
 const VariantInfo *variant_table(int *count);
 inline bool variant_fits(const VariantInfo &v, int H, int W) {
-  return ((H + v.l - 1) / v.l) * W <= v.kmax * v.nt && 4 * (H + W) <= v.ps * v.nt;
+  return H + 2 <= v.rs && W + 2 <= v.rs && ((H + v.l - 1) / v.l) * W <= v.kmax * v.nt &&
+         4 * (H + W) <= v.ps * v.nt;
 }
 
-size_t persistent_lds_bytes(int hmax, int wmax);
-size_t diffuse_lds_bytes(int hmax, int wmax);
+size_t persistent_lds_bytes(const VariantInfo &v, int hmax, int wmax);
+size_t diffuse_lds_bytes(const VariantInfo &v, int hmax);
 size_t cells_lds_bytes(int hmax, int wmax);
 
 hipError_t launch_persistent_lds(int variant, const dev::RunArgs &a, int grid, size_t lds,

@@ -17,12 +17,12 @@ namespace dev {
 
 // Loads a simulation's substrates into the padded LDS planes (zero ring) and
 // its site states into LDS.
-template <int NT>
-__device__ void load_lattice(const RunArgs &a, const SimDesc &d, float *sub, uint8_t *st) {
-  const int H = d.H, W = d.W, PW = W + 2, PV = (H + 2) * PW, HW = H * W;
+template <int NT, int RS>
+__device__ void load_lattice(const RunArgs &a, const SimDesc &d, float *sub, int PV, uint8_t *st) {
+  const int H = d.H, W = d.W, HW = H * W;
   const float *g = a.sub_pool + d.sub_off;
-  for (int q = threadIdx.x; q < PV; q += NT) {
-    const int pr = q / PW, pc = q - pr * PW;
+  for (int q = threadIdx.x; q < (H + 2) * RS; q += NT) {
+    const int pr = q / RS, pc = q - pr * RS;
     const bool in = pr >= 1 && pr <= H && pc >= 1 && pc <= W;
     const int x = (pr - 1) * W + (pc - 1);
 #pragma unroll
@@ -33,14 +33,14 @@ __device__ void load_lattice(const RunArgs &a, const SimDesc &d, float *sub, uin
   for (int k = threadIdx.x; k < (HW + 3) / 4; k += NT) ls[k] = gs[k];
 }
 
-template <int NT>
-__device__ void store_lattice(const RunArgs &a, const SimDesc &d, const float *sub,
+template <int NT, int RS>
+__device__ void store_lattice(const RunArgs &a, const SimDesc &d, const float *sub, int PV,
                               const uint8_t *st) {
-  const int H = d.H, W = d.W, PW = W + 2, PV = (H + 2) * PW, HW = H * W;
+  const int H = d.H, W = d.W, HW = H * W;
   float *g = a.sub_pool + d.sub_off;
   for (int x = threadIdx.x; x < HW; x += NT) {
     const int r = x / W, c = x - r * W;
-    const int q = (r + 1) * PW + (c + 1);
+    const int q = (r + 1) * RS + (c + 1);
 #pragma unroll
     for (int s = 0; s < 3; ++s) g[s * HW + x] = sub[s * PV + q];
   }
@@ -49,12 +49,16 @@ __device__ void store_lattice(const RunArgs &a, const SimDesc &d, const float *s
   for (int k = threadIdx.x; k < (HW + 3) / 4; k += NT) gs[k] = ls[k];
 }
 
-template <int NT, int L, int KMAX, int PS>
+// LDS of the persistent kernel: Shared | 3 substrate planes of pv_max floats
+// (row stride RS) | site states.  The site-state bytes also keep the planes'
+// over-reads of the last diffusion band in bounds (persistent_lds_bytes).
+template <int NT, int L, int KMAX, int RS, int PS>
 __global__ __launch_bounds__(NT) void persistent_kernel(RunArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   Shared &sm = *reinterpret_cast<Shared *>(smem);
   float *sub = reinterpret_cast<float *>(smem + sizeof(Shared));
-  uint8_t *st = reinterpret_cast<uint8_t *>(sub + 3 * a.pv_max);
+  const int PV = a.pv_max;
+  uint8_t *st = reinterpret_cast<uint8_t *>(sub + 3 * PV);
   int32_t *pfx = a.scratch + (int64_t)blockIdx.x * a.scratch_stride;
   uint64_t *ovf = a.ovf + (int64_t)blockIdx.x * a.ovf_stride;
   const int tid = threadIdx.x;
@@ -73,8 +77,8 @@ __global__ __launch_bounds__(NT) void persistent_kernel(RunArgs a) {
       __syncthreads();
       continue;
     }
-    const int H = d.H, W = d.W, PW = W + 2, PV = (H + 2) * PW;
-    load_lattice<NT>(a, d, sub, st);
+    const int H = d.H, W = d.W;
+    load_lattice<NT, RS>(a, d, sub, PV, st);
     if (tid == 0) {
       sm.n = hd->n;
       sm.cur = hd->cur;
@@ -87,20 +91,27 @@ __global__ __launch_bounds__(NT) void persistent_kernel(RunArgs a) {
     const RngKey key{a.seed, a.epoch, d.sim_id};
     uint32_t updates = 0u;
     uint64_t updates64 = 0u;
+    int roi[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) roi[q] = sm.roi[q];
+    uint32_t jac[KMAX], fin[KMAX];
+    plan_diffusion<NT, L, KMAX>(roi, jac, fin, sm);
     for (uint32_t k = 0; k < nsteps; ++k) {
       const uint32_t t = step0 + k;
       if (t % 128u == 0u) rebuild_list<NT>(st, H, W, slots, d.cap, pfx, sm);
-      if (t % 32u == 0u) find_roi<NT>(st, H, W, sm);
-      int roi[5];
+      if (t % 32u == 0u) {
+        find_roi<NT>(st, H, W, sm);
 #pragma unroll
-      for (int q = 0; q < 5; ++q) roi[q] = sm.roi[q];
-      diffuse<NT, L, KMAX, PS>(a.p, sub, PW, PV, roi, updates);
+        for (int q = 0; q < 5; ++q) roi[q] = sm.roi[q];
+        plan_diffusion<NT, L, KMAX>(roi, jac, fin, sm);
+      }
+      diffuse<NT, L, KMAX, RS, PS>(a.p, sub, PV, roi, jac, fin, updates);
       const float dose = dose_at(a, d.prot, t);
-      cells_step<NT>(a.p, key, t, dose, SubLds{sub, PW, PV}, st, H, W, slots, d.cap, ovf, sm);
+      cells_step<NT>(a.p, key, t, dose, SubLds<RS>{sub, PV}, st, H, W, slots, d.cap, ovf, sm);
       if ((k & 1023u) == 1023u) { updates64 += updates; updates = 0u; }
     }
     updates64 += updates;
-    store_lattice<NT>(a, d, sub, st);
+    store_lattice<NT, RS>(a, d, sub, PV, st);
     const uint64_t tot = wg_reduce_sum_u64<NT>(updates64, sm.red);
     if (tid == 0) {
       hd->step = step0 + nsteps;
@@ -139,18 +150,21 @@ __global__ __launch_bounds__(NT) void roi_kernel(RunArgs a) {
   if (threadIdx.x < 5) a.hdr[sim].roi[threadIdx.x] = sm.roi[threadIdx.x];
 }
 
-template <int NT, int L, int KMAX, int PS>
+// LDS: Shared | 3 planes of pv_max floats (row stride RS) | L*RS floats of
+// slack for the last band's over-reads.
+template <int NT, int L, int KMAX, int RS, int PS>
 __global__ __launch_bounds__(NT) void diffuse_kernel(RunArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   Shared &sm = *reinterpret_cast<Shared *>(smem);
   float *sub = reinterpret_cast<float *>(smem + sizeof(Shared));
+  const int PV = a.pv_max;
   const int sim = blockIdx.x;
   if (!sim_active(a, sim)) return;
   const SimDesc d = a.desc[sim];
   int roi[5];
   for (int q = 0; q < 5; ++q) roi[q] = a.hdr[sim].roi[q];
   if (roi[2] == 0 || roi[3] == 0) return;
-  const int H = d.H, W = d.W, PW = W + 2, PV = (H + 2) * PW, HW = H * W;
+  const int H = d.H, W = d.W, HW = H * W;
   // Temporary grid: padded rows r0-1 .. r0+h+2, cols c0-1 .. c0+w+2.
   const int R0 = roi[0] - 1, C0 = roi[1] - 1, th = roi[2] + 4, tw = roi[3] + 4;
   const float *g = a.sub_pool + d.sub_off;
@@ -159,17 +173,19 @@ __global__ __launch_bounds__(NT) void diffuse_kernel(RunArgs a) {
     const bool in = pr >= 1 && pr <= H && pc >= 1 && pc <= W;
     const int x = (pr - 1) * W + (pc - 1);
 #pragma unroll
-    for (int s = 0; s < 3; ++s) sub[s * PV + pr * PW + pc] = in ? g[s * HW + x] : 0.f;
+    for (int s = 0; s < 3; ++s) sub[s * PV + pr * RS + pc] = in ? g[s * HW + x] : 0.f;
   }
-  // diffuse() rewrites ring and masked sites in place: every load must land first.
+  uint32_t jac[KMAX], fin[KMAX];
+  // Also orders the loads above before diffuse() rewrites rings in place.
+  plan_diffusion<NT, L, KMAX>(roi, jac, fin, sm);
   __syncthreads();
   uint32_t updates = 0u;
-  diffuse<NT, L, KMAX, PS>(a.p, sub, PW, PV, roi, updates);
+  diffuse<NT, L, KMAX, RS, PS>(a.p, sub, PV, roi, jac, fin, updates);
   float *gw = a.sub_pool + d.sub_off;
   for (int q = threadIdx.x; q < roi[2] * roi[3]; q += NT) {
     const int r = roi[0] + q / roi[3], c = roi[1] + q % roi[3];
 #pragma unroll
-    for (int s = 0; s < 3; ++s) gw[s * HW + r * W + c] = sub[s * PV + (r + 1) * PW + (c + 1)];
+    for (int s = 0; s < 3; ++s) gw[s * HW + r * W + c] = sub[s * PV + (r + 1) * RS + (c + 1)];
   }
   const uint64_t tot = wg_reduce_sum_u64<NT>(updates, sm.red);
   if (threadIdx.x == 0) atomicAdd((unsigned long long *)&a.hdr[sim].site_substeps,
@@ -251,53 +267,41 @@ namespace {
 
 size_t shared_bytes() { return sizeof(dev::Shared); }
 
-template <int NT, int L, int KMAX, int PS>
-hipError_t launch_persistent_t(const dev::RunArgs &a, int grid, size_t lds, hipStream_t s) {
-  auto k = dev::persistent_kernel<NT, L, KMAX, PS>;
+template <typename K>
+hipError_t launch_lds(K k, int grid, int nt, size_t lds, hipStream_t s, const dev::RunArgs &a) {
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k, dim3(grid), dim3(NT), lds, s, a);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(nt), lds, s, a);
   return hipGetLastError();
 }
 
-template <int NT, int L, int KMAX, int PS>
-hipError_t launch_diffuse_t(const dev::RunArgs &a, int grid, size_t lds, hipStream_t s) {
-  auto k = dev::diffuse_kernel<NT, L, KMAX, PS>;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k, dim3(grid), dim3(NT), lds, s, a);
+template <int NT>
+hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds, hipStream_t s) {
+  if (cells) hipLaunchKernelGGL(dev::cells_kernel<NT>, dim3(nsims), dim3(NT), lds, s, a);
+  else hipLaunchKernelGGL(dev::roi_kernel<NT>, dim3(nsims), dim3(NT), lds, s, a);
   return hipGetLastError();
-}
-
-template <int NT, int L, int KMAX, int PS>
-int variant_occupancy(size_t lds) {
-  int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &nb, reinterpret_cast<const void *>(dev::persistent_kernel<NT, L, KMAX, PS>), NT,
-          lds) != hipSuccess)
-    return 0;
-  return nb;
 }
 
 }  // namespace
 
-// Kernel variants: workgroup size, diffusion band height, items per thread and
-// ring-save slots per thread.
-#define EMT6RO_VARIANTS(X)  \
-  X(0, 256, 4, 3, 2)        \
-  X(1, 256, 2, 6, 2)        \
-  X(2, 256, 8, 2, 2)        \
-  X(3, 512, 4, 2, 1)        \
-  X(4, 512, 8, 1, 1)        \
-  X(5, 128, 4, 6, 4)        \
-  X(6, 256, 4, 7, 3)        \
-  X(7, 512, 4, 4, 2)
+// Kernel variants: id, workgroup size, diffusion band rows, items per thread,
+// ring-save slots per thread, LDS row stride (lattices up to stride-2 wide).
+#define EMT6RO_VARIANTS(X)   \
+  X(0, 128, 4, 6, 4, 56)     \
+  X(1, 256, 4, 3, 2, 56)     \
+  X(2, 128, 8, 3, 4, 56)     \
+  X(3, 256, 2, 6, 2, 56)     \
+  X(4, 256, 8, 2, 2, 56)     \
+  X(5, 128, 4, 3, 3, 36)     \
+  X(6, 64, 4, 5, 5, 36)      \
+  X(7, 256, 8, 4, 3, 88)     \
+  X(8, 512, 4, 4, 2, 88)     \
+  X(9, 256, 4, 8, 3, 88)
 
 const VariantInfo *variant_table(int *count) {
   static const VariantInfo table[] = {
-#define X(id, nt, l, kmax, ps) {id, nt, l, kmax, ps},
+#define X(id, nt, l, kmax, ps, rs) {id, nt, l, kmax, ps, rs},
       EMT6RO_VARIANTS(X)
 #undef X
   };
@@ -305,15 +309,16 @@ const VariantInfo *variant_table(int *count) {
   return table;
 }
 
-size_t persistent_lds_bytes(int hmax, int wmax) {
-  const size_t pv = (size_t)(hmax + 2) * (wmax + 2);
+size_t persistent_lds_bytes(const VariantInfo &v, int hmax, int wmax) {
+  const size_t planes = 3 * (size_t)v.rs * (hmax + 2) * sizeof(float);
   const size_t stb = ((size_t)hmax * wmax + 15) & ~(size_t)15;
-  return shared_bytes() + 3 * pv * sizeof(float) + stb;
+  const size_t slack = (size_t)v.l * v.rs * sizeof(float) + 16;
+  return shared_bytes() + planes + (stb > slack ? stb : slack);
 }
 
-size_t diffuse_lds_bytes(int hmax, int wmax) {
-  const size_t pv = (size_t)(hmax + 2) * (wmax + 2);
-  return shared_bytes() + 3 * pv * sizeof(float);
+size_t diffuse_lds_bytes(const VariantInfo &v, int hmax) {
+  const size_t planes = 3 * (size_t)v.rs * (hmax + 2) * sizeof(float);
+  return shared_bytes() + planes + (size_t)(v.l + 1) * v.rs * sizeof(float);
 }
 
 size_t cells_lds_bytes(int hmax, int wmax) {
@@ -323,8 +328,8 @@ size_t cells_lds_bytes(int hmax, int wmax) {
 hipError_t launch_persistent_lds(int variant, const dev::RunArgs &a, int grid, size_t lds,
                                  hipStream_t s) {
   switch (variant) {
-#define X(id, nt, l, kmax, ps) \
-  case id: return launch_persistent_t<nt, l, kmax, ps>(a, grid, lds, s);
+#define X(id, nt, l, kmax, ps, rs) \
+  case id: return launch_lds(dev::persistent_kernel<nt, l, kmax, rs, ps>, grid, nt, lds, s, a);
     EMT6RO_VARIANTS(X)
 #undef X
     default: return hipErrorInvalidValue;
@@ -332,55 +337,50 @@ hipError_t launch_persistent_lds(int variant, const dev::RunArgs &a, int grid, s
 }
 
 int persistent_occupancy(int variant, size_t lds) {
+  int nb = 0;
+  hipError_t e = hipErrorInvalidValue;
   switch (variant) {
-#define X(id, nt, l, kmax, ps) \
-  case id: return variant_occupancy<nt, l, kmax, ps>(lds);
+#define X(id, nt, l, kmax, ps, rs)                                                        \
+  case id:                                                                                \
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(                                     \
+        &nb, reinterpret_cast<const void *>(dev::persistent_kernel<nt, l, kmax, rs, ps>), \
+        nt, lds);                                                                         \
+    break;
     EMT6RO_VARIANTS(X)
 #undef X
-    default: return 0;
+    default: break;
   }
+  return e == hipSuccess ? nb : 0;
 }
 
 hipError_t launch_step(int variant, const dev::RunArgs &a, int nsims, int hmax, int wmax,
                        hipStream_t s) {
   int cnt = 0;
-  const VariantInfo *v = variant_table(&cnt);
+  const VariantInfo *vt = variant_table(&cnt);
   if (variant < 0 || variant >= cnt) return hipErrorInvalidValue;
-  const int nt = v[variant].nt;
-  // ROI (every 32 steps, decided per simulation inside the kernel).
-  {
-    const size_t lds = cells_lds_bytes(hmax, wmax);
-    if (nt == 512)
-      hipLaunchKernelGGL(dev::roi_kernel<512>, dim3(nsims), dim3(512), lds, s, a);
-    else if (nt == 128)
-      hipLaunchKernelGGL(dev::roi_kernel<128>, dim3(nsims), dim3(128), lds, s, a);
-    else
-      hipLaunchKernelGGL(dev::roi_kernel<256>, dim3(nsims), dim3(256), lds, s, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  {
-    const size_t lds = diffuse_lds_bytes(hmax, wmax);
-    hipError_t e = hipErrorInvalidValue;
-    switch (variant) {
-#define X(id, nt_, l, kmax, ps) \
-  case id: e = launch_diffuse_t<nt_, l, kmax, ps>(a, nsims, lds, s); break;
-      EMT6RO_VARIANTS(X)
-#undef X
-      default: break;
+  const VariantInfo &v = vt[variant];
+  const size_t small = cells_lds_bytes(hmax, wmax);
+  auto small_launch = [&](bool cells) -> hipError_t {
+    switch (v.nt) {
+      case 64: return launch_small<64>(cells, a, nsims, small, s);
+      case 128: return launch_small<128>(cells, a, nsims, small, s);
+      case 512: return launch_small<512>(cells, a, nsims, small, s);
+      default: return launch_small<256>(cells, a, nsims, small, s);
     }
-    if (e != hipSuccess) return e;
+  };
+  // ROI (every 32 steps, decided per simulation inside the kernel).
+  hipError_t e = small_launch(false);
+  if (e != hipSuccess) return e;
+  const size_t lds = diffuse_lds_bytes(v, hmax);
+  switch (variant) {
+#define X(id, nt, l, kmax, ps, rs) \
+  case id: e = launch_lds(dev::diffuse_kernel<nt, l, kmax, rs, ps>, nsims, nt, lds, s, a); break;
+    EMT6RO_VARIANTS(X)
+#undef X
+    default: e = hipErrorInvalidValue; break;
   }
-  {
-    const size_t lds = cells_lds_bytes(hmax, wmax);
-    if (nt == 512)
-      hipLaunchKernelGGL(dev::cells_kernel<512>, dim3(nsims), dim3(512), lds, s, a);
-    else if (nt == 128)
-      hipLaunchKernelGGL(dev::cells_kernel<128>, dim3(nsims), dim3(128), lds, s, a);
-    else
-      hipLaunchKernelGGL(dev::cells_kernel<256>, dim3(nsims), dim3(256), lds, s, a);
-    return hipGetLastError();
-  }
+  if (e != hipSuccess) return e;
+  return small_launch(true);
 }
 
 hipError_t launch_count_living(const SimDesc *desc, const uint8_t *st_pool, uint32_t *out,

@@ -15,7 +15,8 @@ namespace emt6ro {
 namespace dev {
 
 // Ready-to-divide entries kept in LDS per step; more spill to global memory.
-constexpr int kReadyLds = 64;
+constexpr int kReadyLds = 56;
+constexpr int kRedInts = 128;
 constexpr uint32_t kNoWinner = 0xffffffffu;
 
 // Workgroup scalars in LDS.
@@ -24,7 +25,7 @@ struct alignas(16) Shared {
   int32_t n, cur, ready_cnt, sim;
   int32_t roi[5];
   uint32_t winner, divisions, status;
-  int32_t red[64];            // reduction scratch (up to 16 waves x 4 values)
+  int32_t red[kRedInts];      // reductions and diffusion planning scratch
 };
 
 // Kernel-wide arguments shared by all kernels of a run.
@@ -256,9 +257,20 @@ __device__ void find_roi(const uint8_t *st, int H, int W, Shared &sm) {
 // diffusion.cu:16-30); all terms are multiples of 1/4, so this is the integer
 // test a^2 + b^2 >= 4 dist^2 with a = 2(i+1)-(h+2), b = 2(j+1)-(w+2).
 //
-// Work split: the mask box is cut into bands of L rows; an item is one
-// column of one band, and a thread walks its item down the rows with a
-// rolling 3x3 window, so lanes of a wave read consecutive LDS words.
+// Work split: the mask box is cut into bands of L rows.  An item is one
+// column of one band; a band's items cover the columns where the band has
+// unmasked sites (the union of the disc's chords over its rows), so lanes of
+// a wave take consecutive columns and read consecutive LDS words.  Items are
+// dealt to the waves in equal contiguous blocks.  The final substep has its
+// own items over the ROI.  Items are planned when the ROI changes and kept in
+// registers: bits 0-7 column, 8-15 first row (mask-box coordinates), 16-31
+// the mask of rows the item updates.
+//
+// The LDS row stride RS is a compile-time constant, so the rows an item
+// walks are immediate offsets from one address.  An item of the last band may
+// read up to L-1 rows below the temporary grid; those rows only feed results
+// that are not stored, and the LDS layout keeps them inside the allocation.
+//
 // Stencil, per substrate, in this order (no other fused operations):
 //   V(c) = up(c) + down(c),  o = V(c) + (left + right),  d = V(c-1) + V(c+1)
 //   L = fma(d, 1/sqrt2, o);  L = fma(-F, x, L);  x' = fma(k, L, x)
@@ -268,9 +280,89 @@ __device__ __forceinline__ bool masked(int i, int j, int hm, int wm, int d4) {
   return a * a + b * b >= d4;
 }
 
-template <int NT, int L, int KMAX, int PS>
-__device__ void diffuse(const DevParams &p, float *sub, int PW, int PV, const int roi[5],
-                        uint32_t &updates) {
+// Item index for slot k of this thread when n items are dealt to the NT/64
+// waves in equal contiguous blocks; -1 if the slot is empty.
+template <int NT>
+__device__ __forceinline__ int dealt_item(int k, int n) {
+  constexpr int NW = NT / 64;
+  const int Q = (n + NW - 1) / NW;
+  const int off = (int)(threadIdx.x & 63) + 64 * k;
+  const int q = (int)(threadIdx.x >> 6) * Q + off;
+  return (off < Q && q < n) ? q : -1;
+}
+
+__device__ __forceinline__ uint32_t pack_item(int j, int i0, uint32_t mask) {
+  return (uint32_t)j | ((uint32_t)i0 << 8) | (mask << 16);
+}
+
+// Plans this thread's Jacobi (unmasked mask-box sites) and final-substep
+// (ROI sites) items for `roi`.  Uses sm.red as scratch; ends with a barrier.
+template <int NT, int L, int KMAX>
+__device__ void plan_diffusion(const int roi[5], uint32_t jac[KMAX], uint32_t fin[KMAX],
+                               Shared &sm) {
+  const int h = roi[2], w = roi[3], dist = roi[4];
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) { jac[k] = 0u; fin[k] = 0u; }
+  if (h == 0 || w == 0) return;
+  const int hm = h + 2, wm = w + 2, d4 = 4 * dist * dist;
+  const int nb = (hm + L - 1) / L;
+  int32_t *lo = sm.red, *cnt = sm.red + kRedInts / 2;
+  const int tid = threadIdx.x;
+  for (int b = tid; b < nb; b += NT) {
+    int a = wm, e = 0;
+    for (int l = 0; l < L; ++l) {
+      const int i = b * L + l;
+      if (i >= hm) break;
+      const int ai = 2 * (i + 1) - hm;
+      const int rem = d4 - ai * ai;
+      if (rem <= 0) continue;
+      for (int j = 0; j < wm; ++j) {
+        const int bj = 2 * (j + 1) - wm;
+        if (bj * bj < rem) { a = min(a, j); e = max(e, j + 1); }
+      }
+    }
+    lo[b] = a;
+    cnt[b] = e > a ? e - a : 0;
+  }
+  __syncthreads();
+  int n = 0;
+  for (int b = 0; b < nb; ++b) n += cnt[b];
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) {
+    const int q = dealt_item<NT>(k, n);
+    if (q < 0) continue;
+    int b = 0, base = 0;
+    while (base + cnt[b] <= q) { base += cnt[b]; ++b; }
+    const int j = lo[b] + (q - base), i0 = b * L;
+    uint32_t m = 0u;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const int i = i0 + l;
+      if (i < hm && !masked(i, j, hm, wm, d4)) m |= 1u << l;
+    }
+    jac[k] = pack_item(j, i0, m);
+  }
+  const int nbf = (h + L - 1) / L;
+  const int nf = nbf * w;
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) {
+    const int q = dealt_item<NT>(k, nf);
+    if (q < 0) continue;
+    const int band = q / w, j = 1 + (q - band * w), i0 = 1 + band * L;
+    uint32_t m = 0u;
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+      if (i0 + l <= h) m |= 1u << l;
+    fin[k] = pack_item(j, i0, m);
+  }
+  __syncthreads();
+}
+
+// One diffusion step on the padded lattice in LDS (planes PV floats apart,
+// row stride RS).  `jac` and `fin` come from plan_diffusion for the same ROI.
+template <int NT, int L, int KMAX, int RS, int PS>
+__device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5],
+                        const uint32_t jac[KMAX], const uint32_t fin[KMAX], uint32_t &updates) {
   const int r0 = roi[0], c0 = roi[1], h = roi[2], w = roi[3], dist = roi[4];
   if (h == 0 || w == 0) return;
   const int tid = threadIdx.x;
@@ -296,7 +388,7 @@ __device__ void diffuse(const DevParams &p, float *sub, int PW, int PV, const in
       else if ((q -= w + 2) < w + 2) { pr = R1 - 1; pc = C0 + 1 + q; outer = false; }
       else if ((q -= w + 2) < h) { pr = R0 + 2 + q; pc = C0 + 1; outer = false; }
       else { q -= h; pr = R0 + 2 + q; pc = C1 - 1; outer = false; }
-      const int idx = pr * PW + pc;
+      const int idx = pr * RS + pc;
       sidx[k] = idx;
 #pragma unroll
       for (int s = 0; s < 3; ++s) sv[k][s] = sub[s * PV + idx];
@@ -312,32 +404,9 @@ __device__ void diffuse(const DevParams &p, float *sub, int PW, int PV, const in
   for (int q = tid; q < h * w; q += NT) {
     const int i = 1 + q / w, j = 1 + q % w;
     if (masked(i, j, hm, wm, d4)) {
-      const int idx = (r0 + i) * PW + (c0 + j);
+      const int idx = (r0 + i) * RS + (c0 + j);
 #pragma unroll
       for (int s = 0; s < 3; ++s) sub[s * PV + idx] = p.ext[s];
-    }
-  }
-  // Items: band b (rows b*L .. b*L+L-1 of the mask box) x column j.
-  const int nb = (hm + L - 1) / L;
-  const int items = nb * wm;
-  uint32_t act[KMAX], fin[KMAX];
-  int base[KMAX];
-#pragma unroll
-  for (int k = 0; k < KMAX; ++k) {
-    const int q = tid + k * NT;
-    act[k] = 0u; fin[k] = 0u; base[k] = 0;
-    if (q < items) {
-      const int b = q / wm, j = q - b * wm;
-      const int i0 = b * L;
-      base[k] = (i0 << 16) | j;
-#pragma unroll
-      for (int l = 0; l < L; ++l) {
-        const int i = i0 + l;
-        if (i < hm) {
-          if (!masked(i, j, hm, wm, d4)) act[k] |= 1u << l;
-          if (i >= 1 && i <= h && j >= 1 && j <= w) fin[k] |= 1u << l;
-        }
-      }
     }
   }
   const int S = p.substeps;
@@ -345,35 +414,32 @@ __device__ void diffuse(const DevParams &p, float *sub, int PW, int PV, const in
   {
     uint32_t na = 0, nf = 0;
 #pragma unroll
-    for (int k = 0; k < KMAX; ++k) { na += __popc(act[k]); nf += __popc(fin[k]); }
+    for (int k = 0; k < KMAX; ++k) { na += __popc(jac[k] >> 16); nf += __popc(fin[k] >> 16); }
     updates += na * (uint32_t)(total - 1) + nf;
   }
   const float R2 = (float)0.70710678118654752440;
   const float F = (float)(4.0 + (double)(float)(2.0 * 1.41421356237309504880));
-  const int rmax = R1;  // last padded row of the temporary grid
+  // Mask-box (0, 0) is padded (r0, c0).
+  const int origin = r0 * RS + c0;
   __syncthreads();
   for (int step = 0; step < total; ++step) {
     const bool final_sub = (step == total - 1);
     float res[KMAX][L][3];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
-      const uint32_t m = final_sub ? fin[k] : act[k];
-      if (m == 0u) continue;
-      const int i0 = base[k] >> 16, j = base[k] & 0xffff;
-      const int pc = c0 + j;
-      const int pr0 = r0 + i0;
+      const uint32_t it = final_sub ? fin[k] : jac[k];
+      if ((it >> 16) == 0u) continue;
+      const int j = (int)(it & 0xffu), i0 = (int)((it >> 8) & 0xffu);
+      const int a0 = origin + (i0 - 1) * RS + j;
 #pragma unroll
       for (int s = 0; s < 3; ++s) {
-        const float *P = sub + s * PV;
+        const float *P = sub + s * PV + a0;
         const float kk = p.k[s];
-        const float *up = P + (pr0 - 1) * PW + pc;
-        const float *mid = P + pr0 * PW + pc;
-        float ul = up[-1], uc = up[0], ur = up[1];
-        float ml = mid[-1], mc = mid[0], mr = mid[1];
+        float ul = P[-1], uc = P[0], ur = P[1];
+        float ml = P[RS - 1], mc = P[RS], mr = P[RS + 1];
 #pragma unroll
         for (int l = 0; l < L; ++l) {
-          const int prd = min(pr0 + l + 1, rmax);
-          const float *dn = P + prd * PW + pc;
+          const float *dn = P + (l + 2) * RS;
           const float dl = dn[-1], dc = dn[0], dr = dn[1];
           const float vl = ul + dl, vc = uc + dc, vr = ur + dr;
           const float o = vc + (ml + mr);
@@ -389,15 +455,16 @@ __device__ void diffuse(const DevParams &p, float *sub, int PW, int PV, const in
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
-      const uint32_t m = final_sub ? fin[k] : act[k];
+      const uint32_t it = final_sub ? fin[k] : jac[k];
+      const uint32_t m = it >> 16;
       if (m == 0u) continue;
-      const int i0 = base[k] >> 16, j = base[k] & 0xffff;
-      const int idx0 = (r0 + i0) * PW + (c0 + j);
+      const int j = (int)(it & 0xffu), i0 = (int)((it >> 8) & 0xffu);
+      float *P = sub + origin + i0 * RS + j;
 #pragma unroll
       for (int l = 0; l < L; ++l) {
         if (m & (1u << l)) {
 #pragma unroll
-          for (int s = 0; s < 3; ++s) sub[s * PV + idx0 + l * PW] = res[k][l][s];
+          for (int s = 0; s < 3; ++s) P[s * PV + l * RS] = res[k][l][s];
         }
       }
     }
@@ -428,10 +495,11 @@ __device__ void diffuse(const DevParams &p, float *sub, int PW, int PV, const in
 // its OCCUPIED bits with 0x80 set, so neighbour counts still see it occupied;
 // the marks are cleared before the division is applied.
 // ---------------------------------------------------------------------------
+template <int RS>
 struct SubLds {
   float *sub;
-  int PW, PV;
-  __device__ float *at(int s, int r, int c) const { return sub + s * PV + (r + 1) * PW + (c + 1); }
+  int PV;
+  __device__ float *at(int s, int r, int c) const { return sub + s * PV + (r + 1) * RS + (c + 1); }
 };
 struct SubGlobal {
   float *sub;

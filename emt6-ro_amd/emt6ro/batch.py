@@ -72,6 +72,15 @@ class Batch:
     def reset(self, epoch=0):
         check(lib.emt6ro_batch_reset(self._h, int(epoch)))
 
+    def set_variant(self, rs, variant):
+        """Kernel variant for the persistent launches of lattices needing LDS
+        row stride rs (36, 56 or 88); -1 restores the default."""
+        check(lib.emt6ro_batch_set_variant(self._h, int(rs), int(variant)))
+
+    def set_doses(self, doses):
+        d = np.ascontiguousarray(doses, np.float32)
+        check(lib.emt6ro_batch_set_doses(self._h, _p(d, C.c_float)))
+
     def __del__(self):
         if getattr(self, "_h", None) and self._h.value:
             lib.emt6ro_batch_destroy(self._h)
@@ -79,11 +88,19 @@ class Batch:
 
 
 def variants():
-    """Compiled kernel shapes: dicts of id, nt, l, kmax, ps."""
+    """Compiled kernel shapes: dicts of id, nt (workgroup size), l (band
+    rows), kmax (items per thread), ps (ring slots per thread), rs (LDS row
+    stride)."""
     n = lib.emt6ro_variant_info(-1, None)
     out = []
     for i in range(n):
-        info = (C.c_int32 * 5)()
+        info = (C.c_int32 * 6)()
         check(lib.emt6ro_variant_info(i, info))
-        out.append(dict(zip(("id", "nt", "l", "kmax", "ps"), list(info))))
+        out.append(dict(zip(("id", "nt", "l", "kmax", "ps", "rs"), list(info))))
     return out
+
+
+def fits(v, H, W):
+    """Whether variant v (a dict from variants()) takes bordered H x W lattices."""
+    return (H + 2 <= v["rs"] and W + 2 <= v["rs"] and
+            -(-H // v["l"]) * W <= v["kmax"] * v["nt"] and 4 * (H + W) <= v["ps"] * v["nt"])

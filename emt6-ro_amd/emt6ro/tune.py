@@ -1,0 +1,65 @@
+"""Autotuner for the persistent kernel.
+
+For each lattice size class of a batch (LDS row stride 36 / 56 / 88), times
+every compiled kernel variant that fits (workgroup size, diffusion band rows
+= substep unroll depth, items per thread) on a short calibration run of the
+batch itself, keeps the fastest, and restores the batch to its initial
+state.  Choices are cached per (class, lattice size, batch size) in a JSON
+file so later batches of the same shape skip the calibration.
+"""
+import json
+import os
+
+from .batch import fits, variants
+
+CACHE = os.environ.get("EMT6RO_TUNE_CACHE",
+                       os.path.join(os.path.expanduser("~"), ".cache", "emt6ro_amd", "tune.json"))
+
+
+def _load():
+    try:
+        with open(CACHE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _save(d):
+    try:
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        with open(CACHE, "w") as f:
+            json.dump(d, f, indent=1, sort_keys=True)
+    except OSError:
+        pass
+
+
+def autotune(batch, classes, steps=150, use_cache=True, log=None):
+    """classes: {rs: (hmax, wmax)} of the batch's size classes.
+    Returns {rs: {"variant": id, "ms": {id: ms}}}; the batch is left reset
+    (epoch 0) with the chosen variants set."""
+    cache = _load() if use_cache else {}
+    out = {}
+    vs = variants()
+    for rs, (hmax, wmax) in sorted(classes.items()):
+        key = f"rs{rs}_h{hmax}_w{wmax}_n{batch.n}"
+        if use_cache and key in cache:
+            batch.set_variant(rs, cache[key]["variant"])
+            out[rs] = cache[key]
+            continue
+        timings = {}
+        for v in vs:
+            if not fits(v, hmax, wmax) or v["rs"] < hmax + 2:
+                continue
+            batch.reset(0)
+            batch.set_variant(rs, v["id"])
+            timings[v["id"]] = batch.run(steps, path="persistent")
+            if log:
+                log(f"[tune] rs={rs} variant={v} {timings[v['id']]:.2f} ms")
+        best = min(timings, key=timings.get)
+        batch.set_variant(rs, best)
+        out[rs] = {"variant": best, "ms": timings}
+        cache[key] = out[rs]
+    batch.reset(0)
+    if use_cache:
+        _save(cache)
+    return out

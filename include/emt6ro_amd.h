@@ -161,9 +161,15 @@ void emt6ro_batch_destroy(emt6ro_batch *b);
 
 /* ---- Kernel variants and tuning ----------------------------------------- */
 
-/* info[0..4] = id, workgroup size, diffusion band rows, items per thread,
- * ring slots per thread.  Returns the number of variants if idx < 0. */
+/* info[0..5] = id, workgroup size, diffusion band rows, items per thread,
+ * ring slots per thread, LDS row stride (fits lattices up to stride - 2 on
+ * a side).  Returns the number of variants if idx < 0. */
 int emt6ro_variant_info(int32_t idx, int32_t *info);
+/* Variant the persistent path uses for simulations whose lattice needs LDS
+ * row stride `rs` (36, 56 or 88); -1 restores the default choice. */
+int emt6ro_batch_set_variant(emt6ro_batch *b, int32_t rs, int32_t variant);
+/* Replaces the batch's dose table (n_prot x n_slots, as at creation). */
+int emt6ro_batch_set_doses(emt6ro_batch *b, const float *prot_doses);
 /* Workgroups per CU the persistent kernel of `variant` reaches for lattices
  * up to hmax x wmax (0 if it does not fit). */
 int emt6ro_variant_occupancy(int32_t variant, int32_t hmax, int32_t wmax);

@@ -1,0 +1,75 @@
+"""Uneven batches on the GPU: lattices of 33, 53 and 81 with different
+horizons in one batch (one persistent launch per size class), checked bit for
+bit against the oracle, and the autotuner's choice checked not to change
+results."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import golden
+from test_gpu_parity import compare_states, oracle_state_from_arrays
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_uneven_sizes_and_horizons_match_oracle():
+    import emt6ro.simulation as sim
+    from emt6ro.batch import Batch
+    from emt6ro.synthetic import disk_state
+    params = sim.load_parameters(golden("default-parameters.json"))
+    v = params.values()
+    sizes = [33, 53, 81, 33, 81, 53]
+    radii = [6, 10, 14, 9, 5, 12]
+    tumors, orc_states = [], []
+    for k, (sz, r) in enumerate(zip(sizes, radii)):
+        st, arr = disk_state(params, sz, sz, radius=r, seed=50 + k)
+        tumors.append(st)
+        orc_states.append(oracle_state_from_arrays(arr, v))
+    n = len(tumors) * 2
+    tumor_index = np.repeat(np.arange(len(tumors)), 2)
+    horizon = np.array([900, 1500, 600, 1500, 1200, 900, 1500, 1500, 300, 1200, 1500, 600],
+                       np.uint32)
+    doses = np.zeros((2, 480), np.float32)
+    doses[0, 0] = 2.5
+    doses[1, 2] = 1.0
+    prot_index = np.arange(n) % 2
+    b = Batch(params, tumors, tumor_index, prot_index, horizon, doses, 300, 144000,
+              sim_id=np.arange(n), seed=99)
+    log("[uneven] run 1500")
+    b.run(1500)
+    got = b.results().astype(np.int64)
+    cnt = b.counters()
+    assert np.array_equal(cnt["step"].astype(np.int64), horizon.astype(np.int64))
+    for i in range(n):
+        t = int(tumor_index[i])
+        orc = O.OracleBatch(v, [orc_states[t]], doses, [prot_index[i]], [i], 300, 144000)
+        orc.run(int(horizon[i]), seed=99)
+        assert got[i] == orc.counts()[0], (i, got[i], orc.counts()[0])
+        compare_states(b.state(i), orc, 0, f"uneven sim {i} ({sizes[t]}x{sizes[t]})")
+
+
+def test_autotuner_keeps_results():
+    import emt6ro.simulation as sim
+    from emt6ro.batch import Batch
+    from emt6ro.synthetic import disk_state
+    from emt6ro.tune import autotune
+    params = sim.load_parameters(golden("default-parameters.json"))
+    tumors = [disk_state(params, 53, 53, radius=10, seed=1)[0],
+              disk_state(params, 33, 33, radius=8, seed=2)[0]]
+    n = 64
+    b = Batch(params, tumors, np.arange(n) % 2, np.zeros(n, np.int32), np.full(n, 2 ** 32 - 1),
+              np.zeros((1, 480), np.float32), 300, 144000, sim_id=np.arange(n), seed=3)
+    b.run(400)
+    want = b.results()
+    b.reset(0)
+    res = autotune(b, {56: (53, 53), 36: (33, 33)}, steps=60, use_cache=False, log=log)
+    assert set(res) == {56, 36}
+    b.run(400)
+    assert np.array_equal(b.results(), want)
