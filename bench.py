@@ -40,6 +40,10 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the per-step path sample")
     ap.add_argument("--cpu-sims", type=int, default=16)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner")
+    ap.add_argument("--copy-calibrate", action="store_true",
+                    help="only run a 1 GiB device copy (profiler counter calibration)")
+    ap.add_argument("--sims", type=int, default=0,
+                    help="use only the first N simulations (profiling runs)")
     ap.add_argument("--tune-steps", type=int, default=600)
     ap.add_argument("--json-out", default=None)
     return ap.parse_args()
@@ -86,8 +90,24 @@ def cpu_baseline(params, wl, n_sims, hours):
                       f"full {hours} h, CPU oracle (C, FP32, OpenMP over simulations)"}
 
 
+def copy_calibration(gib=1):
+    """A plain device copy of known size, to scale rocprofv3's FETCH_SIZE and
+    WRITE_SIZE counters (run under the profiler with --copy-calibrate)."""
+    import torch
+    n = gib * (1 << 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": n * 4, "copies": 3}), flush=True)
+
+
 def main():
     args = parse()
+    if args.copy_calibrate:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the simulations are "
               "independent, so N GPUs would each run 1/N of the batch.", file=sys.stderr)
@@ -99,6 +119,9 @@ def main():
 
     params = sim.load_parameters(os.path.join(ROOT, "tests", "golden", "default-parameters.json"))
     wl = batched_uniform(params, seed=0)
+    if args.sims:
+        for k in ("tumor_index", "prot_index", "horizon"):
+            wl[k] = wl[k][:args.sims]
     n = len(wl["tumor_index"])
     b = Batch(params, wl["tumors"], wl["tumor_index"], wl["prot_index"], wl["horizon"],
               wl["doses"], 300, wl["prot_length"], sim_id=np.arange(n), seed=0)

@@ -415,11 +415,13 @@ int emt6ro_variant_info(int32_t idx, int32_t *info) {
   return guarded([&] {
     need(idx < cnt && info, "variant index out of range");
     info[0] = v[idx].id;
-    info[1] = v[idx].nt;
-    info[2] = v[idx].l;
-    info[3] = v[idx].kmax;
-    info[4] = v[idx].ps;
-    info[5] = v[idx].rs;
+    info[1] = v[idx].kind;
+    info[2] = v[idx].nt;
+    info[3] = v[idx].l;
+    info[4] = v[idx].kmax;
+    info[5] = v[idx].kf;
+    info[6] = v[idx].ps;
+    info[7] = v[idx].rs;
   });
 }
 

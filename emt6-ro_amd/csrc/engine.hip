@@ -270,12 +270,18 @@ dev::RunArgs Batch::args(uint32_t nsteps) const {
   return a;
 }
 
-int Batch::choose_variant(int hmax, int wmax) {
+int Batch::choose_variant(int hmax, int wmax, int kind) {
   int cnt = 0;
   const VariantInfo *v = variant_table(&cnt);
   int best = -1;
-  for (int i = 0; i < cnt; ++i)
-    if (variant_fits(v[i], hmax, wmax) && (best < 0 || v[i].rs < v[best].rs)) best = i;
+  for (int i = 0; i < cnt; ++i) {
+    if (!variant_fits(v[i], hmax, wmax) || (kind >= 0 && v[i].kind != kind)) continue;
+    if (kind < 0 && persistent_occupancy(i, persistent_lds_bytes(v[i], hmax, wmax)) == 0)
+      continue;
+    if (best < 0 || v[i].rs < v[best].rs ||
+        (v[i].rs == v[best].rs && v[i].kind > v[best].kind))
+      best = i;
+  }
   return best;
 }
 
@@ -329,7 +335,8 @@ void Batch::run(uint32_t nsteps, int path, int variant, uint32_t steps_per_launc
     }
     EMT_HIP(hipEventRecord(ev1_, stream_));
   } else if (path == kPathStep || path == kPathStepGraph) {
-    if (variant < 0) variant = choose_variant(hmax_, wmax_);
+    // The per-step kernels use the banded scheme.
+    if (variant < 0 || vt[variant].kind != 0) variant = choose_variant(hmax_, wmax_, 0);
     EMT_ENFORCE(variant >= 0 && variant_fits(vt[variant], hmax_, wmax_), -1, "Kernel variant ",
                 variant, " does not fit lattices of ", hmax_, " x ", wmax_);
     ensure_scratch(nsims_);

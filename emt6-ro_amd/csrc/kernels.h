@@ -9,19 +9,23 @@
 namespace emt6ro {
 namespace dev { struct RunArgs; }
 
-// A compiled shape of the simulation kernels: workgroup size `nt`, diffusion
-// band rows `l`, diffusion items per thread `kmax`, ring-save slots per
-// thread `ps` and LDS row stride `rs`.  A variant fits lattices of bordered
-// dims H x W when H + 2 <= rs, W + 2 <= rs, ceil(H / l) * W <= kmax * nt and
-// 4 (H + W) <= ps * nt.
+// A compiled shape of the simulation kernels: diffusion scheme `kind`
+// (0 banded in place, 1 ping-pong site-parallel), workgroup size `nt`, band
+// rows `l` (1 for kind 1), diffusion items per thread `kmax` (kind 1: Jacobi
+// sites), final-substep sites per thread `kf` (kind 1), ring-save slots per
+// thread `ps` and LDS row stride `rs`.  It fits lattices of bordered dims
+// H x W when H + 2 <= rs, W + 2 <= rs, 4 (H + W) <= ps * nt and the
+// diffusion items fit: kind 0 ceil(H / l) * W <= kmax * nt; kind 1
+// H * W <= kmax * nt and (H - 2) * (W - 2) <= kf * nt.
 struct VariantInfo {
-  int id, nt, l, kmax, ps, rs;
+  int id, kind, nt, l, kmax, kf, ps, rs;
 };
 
 const VariantInfo *variant_table(int *count);
 inline bool variant_fits(const VariantInfo &v, int H, int W) {
-  return H + 2 <= v.rs && W + 2 <= v.rs && ((H + v.l - 1) / v.l) * W <= v.kmax * v.nt &&
-         4 * (H + W) <= v.ps * v.nt;
+  if (H + 2 > v.rs || W + 2 > v.rs || 4 * (H + W) > v.ps * v.nt) return false;
+  if (v.kind == 0) return ((H + v.l - 1) / v.l) * W <= v.kmax * v.nt;
+  return H * W <= v.kmax * v.nt && (H - 2) * (W - 2) <= v.kf * v.nt;
 }
 
 size_t persistent_lds_bytes(const VariantInfo &v, int hmax, int wmax);

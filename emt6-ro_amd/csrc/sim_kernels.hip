@@ -49,19 +49,21 @@ __device__ void store_lattice(const RunArgs &a, const SimDesc &d, const float *s
   for (int k = threadIdx.x; k < (HW + 3) / 4; k += NT) gs[k] = ls[k];
 }
 
-// LDS of the persistent kernel: Shared | 3 substrate planes of pv_max floats
-// (row stride RS) | site states.  The site-state bytes also keep the planes'
-// over-reads of the last diffusion band in bounds (persistent_lds_bytes).
-template <int NT, int L, int KMAX, int RS, int PS>
-__global__ __launch_bounds__(NT) void persistent_kernel(RunArgs a) {
+// LDS of the persistent kernel: Shared | Diff::kPlanes substrate planes of
+// pv_max floats (row stride RS; buffer A first) | site states.  The
+// site-state bytes also keep the banded scheme's over-reads of its last band
+// in bounds (persistent_lds_bytes).
+template <int NT, int RS, typename Diff, int WPE>
+__global__ __launch_bounds__(NT, WPE) void persistent_kernel(RunArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   Shared &sm = *reinterpret_cast<Shared *>(smem);
   float *sub = reinterpret_cast<float *>(smem + sizeof(Shared));
   const int PV = a.pv_max;
-  uint8_t *st = reinterpret_cast<uint8_t *>(sub + 3 * PV);
+  uint8_t *st = reinterpret_cast<uint8_t *>(sub + Diff::kPlanes * PV);
   int32_t *pfx = a.scratch + (int64_t)blockIdx.x * a.scratch_stride;
   uint64_t *ovf = a.ovf + (int64_t)blockIdx.x * a.ovf_stride;
   const int tid = threadIdx.x;
+  Diff diff;
   for (;;) {
     if (tid == 0) sm.sim = atomicAdd(a.queue, 1);
     __syncthreads();
@@ -94,8 +96,7 @@ __global__ __launch_bounds__(NT) void persistent_kernel(RunArgs a) {
     int roi[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) roi[q] = sm.roi[q];
-    uint32_t jac[KMAX], fin[KMAX];
-    plan_diffusion<NT, L, KMAX>(roi, jac, fin, sm);
+    diff.plan(roi, sm);
     for (uint32_t k = 0; k < nsteps; ++k) {
       const uint32_t t = step0 + k;
       if (t % 128u == 0u) rebuild_list<NT>(st, H, W, slots, d.cap, pfx, sm);
@@ -103,9 +104,9 @@ __global__ __launch_bounds__(NT) void persistent_kernel(RunArgs a) {
         find_roi<NT>(st, H, W, sm);
 #pragma unroll
         for (int q = 0; q < 5; ++q) roi[q] = sm.roi[q];
-        plan_diffusion<NT, L, KMAX>(roi, jac, fin, sm);
+        diff.plan(roi, sm);
       }
-      diffuse<NT, L, KMAX, RS, PS>(a.p, sub, PV, roi, jac, fin, updates);
+      diff.run(a.p, sub, PV, roi, updates);
       const float dose = dose_at(a, d.prot, t);
       cells_step<NT>(a.p, key, t, dose, SubLds<RS>{sub, PV}, st, H, W, slots, d.cap, ovf, sm);
       if ((k & 1023u) == 1023u) { updates64 += updates; updates = 0u; }
@@ -125,6 +126,18 @@ __global__ __launch_bounds__(NT) void persistent_kernel(RunArgs a) {
     __syncthreads();
   }
 }
+
+// Diffusion policy of a variant: kind 0 banded in place (A = band rows,
+// B = items per thread), kind 1 ping-pong site-parallel (A = Jacobi sites,
+// B = final sites per thread).
+template <int KIND, int NT, int A, int B, int RS, int PS>
+struct DiffSel {
+  using T = BandedDiffuser<NT, A, B, RS, PS>;
+};
+template <int NT, int A, int B, int RS, int PS>
+struct DiffSel<1, NT, A, B, RS, PS> {
+  using T = PingPongDiffuser<NT, A, B, RS, PS>;
+};
 
 // ---------------------------------------------------------------------------
 // Per-step path.
@@ -285,23 +298,31 @@ hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds
 
 }  // namespace
 
-// Kernel variants: id, workgroup size, diffusion band rows, items per thread,
-// ring-save slots per thread, LDS row stride (lattices up to stride-2 wide).
-#define EMT6RO_VARIANTS(X)   \
-  X(0, 128, 4, 6, 4, 56)     \
-  X(1, 256, 4, 3, 2, 56)     \
-  X(2, 128, 8, 3, 4, 56)     \
-  X(3, 256, 2, 6, 2, 56)     \
-  X(4, 256, 8, 2, 2, 56)     \
-  X(5, 128, 4, 3, 3, 36)     \
-  X(6, 64, 4, 5, 5, 36)      \
-  X(7, 256, 8, 4, 3, 88)     \
-  X(8, 512, 4, 4, 2, 88)     \
-  X(9, 256, 4, 8, 3, 88)
+// Kernel variants: id, kind (0 banded in place, 1 ping-pong site-parallel),
+// workgroup size, A, B (kind 0: band rows, items per thread; kind 1: Jacobi
+// and final sites per thread), ring-save slots per thread, LDS row stride
+// (lattices up to stride-2 on a side), minimum waves per SIMD the compiler
+// must allow (bounds the registers so that many workgroups fit a CU).
+#define EMT6RO_VARIANTS(X)        \
+  X(0, 0, 128, 4, 6, 4, 56, 1)    \
+  X(1, 0, 256, 4, 3, 2, 56, 1)    \
+  X(2, 0, 128, 8, 3, 4, 56, 1)    \
+  X(3, 0, 256, 2, 6, 2, 56, 1)    \
+  X(4, 1, 512, 6, 6, 1, 56, 4)    \
+  X(5, 1, 256, 12, 11, 2, 56, 2)  \
+  X(6, 0, 128, 4, 3, 3, 36, 1)    \
+  X(7, 0, 64, 4, 5, 5, 36, 1)     \
+  X(8, 1, 256, 5, 5, 2, 36, 4)    \
+  X(9, 1, 512, 3, 2, 1, 36, 4)    \
+  X(10, 0, 256, 8, 4, 3, 88, 1)   \
+  X(11, 0, 512, 4, 4, 2, 88, 1)   \
+  X(12, 0, 256, 4, 8, 3, 88, 1)   \
+  X(13, 1, 512, 6, 6, 1, 56, 2)
 
 const VariantInfo *variant_table(int *count) {
   static const VariantInfo table[] = {
-#define X(id, nt, l, kmax, ps, rs) {id, nt, l, kmax, ps, rs},
+#define X(id, kind, nt, A, B, ps, rs, wpe) \
+  {id, kind, nt, kind == 0 ? A : 1, kind == 0 ? B : A, kind == 0 ? 0 : B, ps, rs},
       EMT6RO_VARIANTS(X)
 #undef X
   };
@@ -310,9 +331,9 @@ const VariantInfo *variant_table(int *count) {
 }
 
 size_t persistent_lds_bytes(const VariantInfo &v, int hmax, int wmax) {
-  const size_t planes = 3 * (size_t)v.rs * (hmax + 2) * sizeof(float);
+  const size_t planes = (v.kind == 0 ? 3 : 6) * (size_t)v.rs * (hmax + 2) * sizeof(float);
   const size_t stb = ((size_t)hmax * wmax + 15) & ~(size_t)15;
-  const size_t slack = (size_t)v.l * v.rs * sizeof(float) + 16;
+  const size_t slack = v.kind == 0 ? (size_t)v.l * v.rs * sizeof(float) + 16 : 0;
   return shared_bytes() + planes + (stb > slack ? stb : slack);
 }
 
@@ -328,8 +349,11 @@ size_t cells_lds_bytes(int hmax, int wmax) {
 hipError_t launch_persistent_lds(int variant, const dev::RunArgs &a, int grid, size_t lds,
                                  hipStream_t s) {
   switch (variant) {
-#define X(id, nt, l, kmax, ps, rs) \
-  case id: return launch_lds(dev::persistent_kernel<nt, l, kmax, rs, ps>, grid, nt, lds, s, a);
+#define X(id, kind, nt, A, B, ps, rs, wpe)                                                      \
+  case id:                                                                                \
+    return launch_lds(                                                                    \
+        dev::persistent_kernel<nt, rs, typename dev::DiffSel<kind, nt, A, B, rs, ps>::T, wpe>, \
+        grid, nt, lds, s, a);
     EMT6RO_VARIANTS(X)
 #undef X
     default: return hipErrorInvalidValue;
@@ -340,11 +364,13 @@ int persistent_occupancy(int variant, size_t lds) {
   int nb = 0;
   hipError_t e = hipErrorInvalidValue;
   switch (variant) {
-#define X(id, nt, l, kmax, ps, rs)                                                        \
-  case id:                                                                                \
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(                                     \
-        &nb, reinterpret_cast<const void *>(dev::persistent_kernel<nt, l, kmax, rs, ps>), \
-        nt, lds);                                                                         \
+#define X(id, kind, nt, A, B, ps, rs, wpe)                                                          \
+  case id:                                                                                    \
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(                                         \
+        &nb,                                                                                  \
+        reinterpret_cast<const void *>(                                                       \
+            dev::persistent_kernel<nt, rs, typename dev::DiffSel<kind, nt, A, B, rs, ps>::T, wpe>), \
+        nt, lds);                                                                             \
     break;
     EMT6RO_VARIANTS(X)
 #undef X
@@ -371,10 +397,15 @@ hipError_t launch_step(int variant, const dev::RunArgs &a, int nsims, int hmax, 
   // ROI (every 32 steps, decided per simulation inside the kernel).
   hipError_t e = small_launch(false);
   if (e != hipSuccess) return e;
+  if (v.kind != 0) return hipErrorInvalidValue;
   const size_t lds = diffuse_lds_bytes(v, hmax);
   switch (variant) {
-#define X(id, nt, l, kmax, ps, rs) \
-  case id: e = launch_lds(dev::diffuse_kernel<nt, l, kmax, rs, ps>, nsims, nt, lds, s, a); break;
+#define X(id, kind, nt, A, B, ps, rs, wpe)                                                         \
+  case id:                                                                                  \
+    if (kind == 0)                                                                          \
+      e = launch_lds(dev::diffuse_kernel<nt, (kind == 0 ? A : 4), (kind == 0 ? B : 1), rs, ps>, \
+                     nsims, nt, lds, s, a);                                                 \
+    break;
     EMT6RO_VARIANTS(X)
 #undef X
     default: e = hipErrorInvalidValue; break;

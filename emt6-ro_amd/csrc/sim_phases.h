@@ -480,6 +480,205 @@ __device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5]
   __syncthreads();
 }
 
+// Banded in-place diffusion as a policy for the persistent kernel.
+template <int NT, int L, int KMAX, int RS, int PS>
+struct BandedDiffuser {
+  static constexpr int kPlanes = 3;  // substrate planes in LDS
+  uint32_t jac[KMAX], fin[KMAX];
+  __device__ void plan(const int roi[5], Shared &sm) { plan_diffusion<NT, L, KMAX>(roi, jac, fin, sm); }
+  __device__ void run(const DevParams &p, float *sub, int PV, const int roi[5], uint32_t &upd) {
+    diffuse<NT, L, KMAX, RS, PS>(p, sub, PV, roi, jac, fin, upd);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Site-parallel diffusion with two LDS buffers.  The same temporary-grid
+// values as diffuse() are set up in buffer A (rings saved to registers), the
+// temporary grid is copied to buffer B, and each substep reads one buffer and
+// writes the other, one lane per updated site, so a substep needs a single
+// barrier and every wave is packed with sites.  Sites not updated by a
+// substep hold the same value in both buffers.  The result lands in A (a
+// copy back covers an odd substep count) and the rings are restored there.
+// Sites are planned when the ROI changes: Jacobi sites are the unmasked
+// sites of the mask box in row-major order, final sites the ROI's; each is
+// kept as its LDS offset in the plane (0xffffffff: none).
+// ---------------------------------------------------------------------------
+template <int NT, int KJ, int KF>
+__device__ void plan_sites(const int roi[5], int RS, uint32_t jac[KJ], uint32_t fin[KF],
+                           Shared &sm) {
+  const int r0 = roi[0], c0 = roi[1], h = roi[2], w = roi[3], dist = roi[4];
+#pragma unroll
+  for (int k = 0; k < KJ; ++k) jac[k] = 0xffffffffu;
+#pragma unroll
+  for (int k = 0; k < KF; ++k) fin[k] = 0xffffffffu;
+  if (h == 0 || w == 0) return;
+  const int hm = h + 2, wm = w + 2, d4 = 4 * dist * dist;
+  int32_t *lo = sm.red, *cnt = sm.red + kRedInts / 2;
+  const int tid = threadIdx.x;
+  // Unmasked columns of row i: the chord b^2 < d4 - a^2, b = 2(j+1) - wm.
+  for (int i = tid; i < hm; i += NT) {
+    const int ai = 2 * (i + 1) - hm;
+    const int rem = d4 - ai * ai;
+    int a = wm, e = 0;
+    if (rem > 0) {
+      for (int j = 0; j < wm; ++j) {
+        const int bj = 2 * (j + 1) - wm;
+        if (bj * bj < rem) { a = min(a, j); e = max(e, j + 1); }
+      }
+    }
+    lo[i] = a;
+    cnt[i] = e > a ? e - a : 0;
+  }
+  __syncthreads();
+  int n = 0;
+  for (int i = 0; i < hm; ++i) n += cnt[i];
+  const int origin = r0 * RS + c0;
+#pragma unroll
+  for (int k = 0; k < KJ; ++k) {
+    const int q = tid + k * NT;
+    if (q >= n) continue;
+    int i = 0, base = 0;
+    while (base + cnt[i] <= q) { base += cnt[i]; ++i; }
+    const int j = lo[i] + (q - base);
+    jac[k] = (uint32_t)(origin + i * RS + j);
+  }
+#pragma unroll
+  for (int k = 0; k < KF; ++k) {
+    const int q = tid + k * NT;
+    if (q >= h * w) continue;
+    const int i = 1 + q / w, j = 1 + q % w;
+    fin[k] = (uint32_t)(origin + i * RS + j);
+  }
+  __syncthreads();
+}
+
+// One substep on the sites `site[]`: src -> dst (both plane sets, PV apart).
+template <int K, int RS>
+__device__ __forceinline__ void pp_substep(const DevParams &p, const float *src, float *dst,
+                                           int PV, const uint32_t site[K]) {
+  const float R2 = (float)0.70710678118654752440;
+  const float F = (float)(4.0 + (double)(float)(2.0 * 1.41421356237309504880));
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (site[k] == 0xffffffffu) continue;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      const float *P = src + s * PV + site[k];
+      const float ul = P[-RS - 1], uc = P[-RS], ur = P[-RS + 1];
+      const float ml = P[-1], mc = P[0], mr = P[1];
+      const float dl = P[RS - 1], dc = P[RS], dr = P[RS + 1];
+      const float vl = ul + dl, vc = uc + dc, vr = ur + dr;
+      const float o = vc + (ml + mr);
+      const float dd = vl + vr;
+      float Lp = fmaf(dd, R2, o);
+      Lp = fmaf(-F, mc, Lp);
+      dst[s * PV + site[k]] = fmaf(p.k[s], Lp, mc);
+    }
+  }
+}
+
+template <int NT, int KJ, int KF, int RS, int PS>
+struct PingPongDiffuser {
+  static constexpr int kPlanes = 6;  // two buffers of three planes
+  uint32_t jac[KJ], fin[KF];
+  __device__ void plan(const int roi[5], Shared &sm) { plan_sites<NT, KJ, KF>(roi, RS, jac, fin, sm); }
+  __device__ void run(const DevParams &p, float *A, int PV, const int roi[5], uint32_t &upd) {
+    const int r0 = roi[0], c0 = roi[1], h = roi[2], w = roi[3], dist = roi[4];
+    if (h == 0 || w == 0) return;
+    float *B = A + 3 * PV;
+    const int tid = threadIdx.x;
+    const int hm = h + 2, wm = w + 2, d4 = 4 * dist * dist;
+    const int R0 = r0 - 1, R1 = r0 + h + 2, C0 = c0 - 1, C1 = c0 + w + 2;
+    const int nper = 4 * w + 4 * h + 16;
+    float sv[PS][3];
+    int sidx[PS];
+#pragma unroll
+    for (int k = 0; k < PS; ++k) {
+      int q = tid + k * NT;
+      sidx[k] = -1;
+      if (q < nper) {
+        int pr, pc;
+        bool outer;
+        if (q < w + 4) { pr = R0; pc = C0 + q; outer = true; }
+        else if ((q -= w + 4) < w + 4) { pr = R1; pc = C0 + q; outer = true; }
+        else if ((q -= w + 4) < h + 2) { pr = R0 + 1 + q; pc = C0; outer = true; }
+        else if ((q -= h + 2) < h + 2) { pr = R0 + 1 + q; pc = C1; outer = true; }
+        else if ((q -= h + 2) < w + 2) { pr = R0 + 1; pc = C0 + 1 + q; outer = false; }
+        else if ((q -= w + 2) < w + 2) { pr = R1 - 1; pc = C0 + 1 + q; outer = false; }
+        else if ((q -= w + 2) < h) { pr = R0 + 2 + q; pc = C0 + 1; outer = false; }
+        else { q -= h; pr = R0 + 2 + q; pc = C1 - 1; outer = false; }
+        const int idx = pr * RS + pc;
+        sidx[k] = idx;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) sv[k][s] = A[s * PV + idx];
+        if (outer) {
+#pragma unroll
+          for (int s = 0; s < 3; ++s) A[s * PV + idx] = 0.f;
+        } else if (masked(pr - r0, pc - c0, hm, wm, d4)) {
+#pragma unroll
+          for (int s = 0; s < 3; ++s) A[s * PV + idx] = p.ext[s];
+        }
+      }
+    }
+    for (int q = tid; q < h * w; q += NT) {
+      const int i = 1 + q / w, j = 1 + q % w;
+      if (masked(i, j, hm, wm, d4)) {
+        const int idx = (r0 + i) * RS + (c0 + j);
+#pragma unroll
+        for (int s = 0; s < 3; ++s) A[s * PV + idx] = p.ext[s];
+      }
+    }
+    __syncthreads();
+    // Temporary grid A -> B.
+    const int th = h + 4, tw = w + 4;
+    for (int q = tid; q < th * tw; q += NT) {
+      const int idx = (R0 + q / tw) * RS + (C0 + q % tw);
+#pragma unroll
+      for (int s = 0; s < 3; ++s) B[s * PV + idx] = A[s * PV + idx];
+    }
+    const int S = p.substeps;
+    const int total = S > 1 ? S : 1;
+    {
+      uint32_t na = 0, nf = 0;
+#pragma unroll
+      for (int k = 0; k < KJ; ++k) na += jac[k] != 0xffffffffu;
+#pragma unroll
+      for (int k = 0; k < KF; ++k) nf += fin[k] != 0xffffffffu;
+      upd += na * (uint32_t)(total - 1) + nf;
+    }
+    __syncthreads();
+    for (int step = 0; step < total - 1; ++step) {
+      const float *src = (step & 1) ? B : A;
+      float *dst = (step & 1) ? A : B;
+      pp_substep<KJ, RS>(p, src, dst, PV, jac);
+      __syncthreads();
+    }
+    {
+      const bool odd = ((total - 1) & 1) != 0;
+      const float *src = odd ? B : A;
+      float *dst = odd ? A : B;
+      pp_substep<KF, RS>(p, src, dst, PV, fin);
+      __syncthreads();
+      if (!odd) {
+        // The final substep wrote B: copy the ROI back.
+        for (int q = tid; q < h * w; q += NT) {
+          const int idx = (r0 + 1 + q / w) * RS + (c0 + 1 + q % w);
+#pragma unroll
+          for (int s = 0; s < 3; ++s) A[s * PV + idx] = B[s * PV + idx];
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PS; ++k) {
+      if (sidx[k] >= 0) {
+#pragma unroll
+        for (int s = 0; s < 3; ++s) A[s * PV + sidx[k]] = sv[k][s];
+      }
+    }
+    __syncthreads();
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Cell update (cellSimulationKernel, simulation.cu:66-110, with Site::step,
 // site.h:62-82) under deterministic rules:

@@ -88,19 +88,23 @@ class Batch:
 
 
 def variants():
-    """Compiled kernel shapes: dicts of id, nt (workgroup size), l (band
-    rows), kmax (items per thread), ps (ring slots per thread), rs (LDS row
-    stride)."""
+    """Compiled kernel shapes: dicts of id, kind (0 banded in place, 1
+    ping-pong site-parallel diffusion), nt (workgroup size), l (band rows),
+    kmax (items or Jacobi sites per thread), kf (final sites per thread),
+    ps (ring slots per thread), rs (LDS row stride)."""
     n = lib.emt6ro_variant_info(-1, None)
     out = []
     for i in range(n):
-        info = (C.c_int32 * 6)()
+        info = (C.c_int32 * 8)()
         check(lib.emt6ro_variant_info(i, info))
-        out.append(dict(zip(("id", "nt", "l", "kmax", "ps", "rs"), list(info))))
+        out.append(dict(zip(("id", "kind", "nt", "l", "kmax", "kf", "ps", "rs"), list(info))))
     return out
 
 
 def fits(v, H, W):
     """Whether variant v (a dict from variants()) takes bordered H x W lattices."""
-    return (H + 2 <= v["rs"] and W + 2 <= v["rs"] and
-            -(-H // v["l"]) * W <= v["kmax"] * v["nt"] and 4 * (H + W) <= v["ps"] * v["nt"])
+    if H + 2 > v["rs"] or W + 2 > v["rs"] or 4 * (H + W) > v["ps"] * v["nt"]:
+        return False
+    if v["kind"] == 0:
+        return -(-H // v["l"]) * W <= v["kmax"] * v["nt"]
+    return H * W <= v["kmax"] * v["nt"] and (H - 2) * (W - 2) <= v["kf"] * v["nt"]

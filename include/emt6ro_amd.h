@@ -161,9 +161,11 @@ void emt6ro_batch_destroy(emt6ro_batch *b);
 
 /* ---- Kernel variants and tuning ----------------------------------------- */
 
-/* info[0..5] = id, workgroup size, diffusion band rows, items per thread,
- * ring slots per thread, LDS row stride (fits lattices up to stride - 2 on
- * a side).  Returns the number of variants if idx < 0. */
+/* info[0..7] = id, diffusion scheme (0 banded in place, 1 ping-pong
+ * site-parallel), workgroup size, band rows, items (or Jacobi sites) per
+ * thread, final-substep sites per thread, ring slots per thread, LDS row
+ * stride (fits lattices up to stride - 2 on a side).  Returns the number of
+ * variants if idx < 0. */
 int emt6ro_variant_info(int32_t idx, int32_t *info);
 /* Variant the persistent path uses for simulations whose lattice needs LDS
  * row stride `rs` (36, 56 or 88); -1 restores the default choice. */
