@@ -138,6 +138,10 @@ template <int NT, int A, int B, int RS, int PS>
 struct DiffSel<1, NT, A, B, RS, PS> {
   using T = PingPongDiffuser<NT, A, B, RS, PS>;
 };
+template <int NT, int A, int B, int RS, int PS>
+struct DiffSel<2, NT, A, B, RS, PS> {
+  using T = MultiBandDiffuser<NT, RS, PS>;
+};
 
 // ---------------------------------------------------------------------------
 // Per-step path.
@@ -298,7 +302,7 @@ hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds
 
 }  // namespace
 
-// Kernel variants: id, kind (0 banded in place, 1 ping-pong site-parallel),
+// Kernel variants: id, kind (0 banded in place, 1 ping-pong site-parallel, 2 banded with the band height chosen per ROI),
 // workgroup size, A, B (kind 0: band rows, items per thread; kind 1: Jacobi
 // and final sites per thread), ring-save slots per thread, LDS row stride
 // (lattices up to stride-2 on a side), minimum waves per SIMD the compiler
@@ -317,12 +321,19 @@ hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds
   X(10, 0, 256, 8, 4, 3, 88, 1)   \
   X(11, 0, 512, 4, 4, 2, 88, 1)   \
   X(12, 0, 256, 4, 8, 3, 88, 1)   \
-  X(13, 1, 512, 6, 6, 1, 56, 2)
+  X(13, 1, 512, 6, 6, 1, 56, 2)   \
+  X(14, 2, 256, 0, 0, 2, 56, 4)   \
+  X(15, 2, 128, 0, 0, 4, 56, 1)   \
+  X(16, 2, 256, 0, 0, 2, 56, 1)   \
+  X(17, 2, 256, 0, 0, 2, 36, 4)   \
+  X(18, 2, 128, 0, 0, 3, 36, 1)   \
+  X(19, 2, 256, 0, 0, 3, 88, 2)
 
 const VariantInfo *variant_table(int *count) {
   static const VariantInfo table[] = {
 #define X(id, kind, nt, A, B, ps, rs, wpe) \
-  {id, kind, nt, kind == 0 ? A : 1, kind == 0 ? B : A, kind == 0 ? 0 : B, ps, rs},
+  {id, kind, nt, kind == 0 ? A : kind == 1 ? 1 : 8,                                  \
+   kind == 0 ? B : kind == 1 ? A : dev::MultiBand<nt, rs>::kSlotsMax, kind == 1 ? B : 0, ps, rs},
       EMT6RO_VARIANTS(X)
 #undef X
   };
@@ -331,9 +342,9 @@ const VariantInfo *variant_table(int *count) {
 }
 
 size_t persistent_lds_bytes(const VariantInfo &v, int hmax, int wmax) {
-  const size_t planes = (v.kind == 0 ? 3 : 6) * (size_t)v.rs * (hmax + 2) * sizeof(float);
+  const size_t planes = (v.kind == 1 ? 6 : 3) * (size_t)v.rs * (hmax + 2) * sizeof(float);
   const size_t stb = ((size_t)hmax * wmax + 15) & ~(size_t)15;
-  const size_t slack = v.kind == 0 ? (size_t)v.l * v.rs * sizeof(float) + 16 : 0;
+  const size_t slack = v.kind != 1 ? (size_t)v.l * v.rs * sizeof(float) + 16 : 0;
   return shared_bytes() + planes + (stb > slack ? stb : slack);
 }
 

@@ -280,15 +280,13 @@ __device__ __forceinline__ bool masked(int i, int j, int hm, int wm, int d4) {
   return a * a + b * b >= d4;
 }
 
-// Item index for slot k of this thread when n items are dealt to the NT/64
-// waves in equal contiguous blocks; -1 if the slot is empty.
+// Item index for slot k of this thread: items fill the waves in order
+// (item q on thread q mod NT, slot q / NT), so waves left without items
+// leave their SIMDs to other workgroups; -1 if the slot is empty.
 template <int NT>
 __device__ __forceinline__ int dealt_item(int k, int n) {
-  constexpr int NW = NT / 64;
-  const int Q = (n + NW - 1) / NW;
-  const int off = (int)(threadIdx.x & 63) + 64 * k;
-  const int q = (int)(threadIdx.x >> 6) * Q + off;
-  return (off < Q && q < n) ? q : -1;
+  const int q = (int)threadIdx.x + k * NT;
+  return q < n ? q : -1;
 }
 
 __device__ __forceinline__ uint32_t pack_item(int j, int i0, uint32_t mask) {
@@ -479,6 +477,124 @@ __device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5]
   }
   __syncthreads();
 }
+
+// Banded in-place diffusion whose band height is chosen per ROI.  For each
+// candidate height the plan counts the band items and picks the height with
+// the fewest wave instructions, estimated as (wave rounds of 64 items) x
+// (27 L + 28) VALU operations per item; items are packed into waves in
+// order (item q on thread q mod NT, round q / NT) so unused waves leave their
+// SIMDs to other workgroups.
+template <int NT, int RS>
+struct MultiBand {
+  static constexpr int kMaxItems(int L) { return ((RS - 2 + L - 1) / L) * (RS - 2); }
+  static constexpr int kSlots(int L) { return (kMaxItems(L) + NT - 1) / NT; }
+  static constexpr int kSlotsMax = kSlots(3);
+};
+
+// Band heights tried per ROI: 3, 4 and 6 rows.
+template <int NT, int RS, int PS>
+struct MultiBandDiffuser {
+  static constexpr int kPlanes = 3;
+  static constexpr int KM = MultiBand<NT, RS>::kSlotsMax;
+  uint32_t jac[KM], fin[KM];
+  int L = 4;
+
+  // Items of band height Lc for the chords in LDS; with item_q >= 0 also
+  // returns that item's band and first column.
+  __device__ static int count_items(const int32_t *chord, int hm, int wm, int Lc, int item_q,
+                                    int &band, int &col) {
+    int n = 0;
+    for (int b = 0; b * Lc < hm; ++b) {
+      int a = wm, e = 0;
+      for (int l = 0; l < Lc && b * Lc + l < hm; ++l) {
+        const int c = chord[b * Lc + l];
+        if (c) { a = min(a, c & 0xffff); e = max(e, c >> 16); }
+      }
+      const int width = e > a ? e - a : 0;
+      if (item_q >= n && item_q < n + width) { band = b; col = a + (item_q - n); }
+      n += width;
+    }
+    return n;
+  }
+
+  __device__ void plan(const int roi[5], Shared &sm) {
+    const int h = roi[2], w = roi[3], dist = roi[4];
+#pragma unroll
+    for (int k = 0; k < KM; ++k) { jac[k] = 0u; fin[k] = 0u; }
+    if (h == 0 || w == 0) return;
+    const int hm = h + 2, wm = w + 2, d4 = 4 * dist * dist;
+    int32_t *chord = sm.red;  // per mask-box row: first | (past-last << 16)
+    for (int i = threadIdx.x; i < hm; i += NT) {
+      const int ai = 2 * (i + 1) - hm;
+      const int rem = d4 - ai * ai;
+      int a = wm, e = 0;
+      if (rem > 0)
+        for (int j = 0; j < wm; ++j) {
+          const int bj = 2 * (j + 1) - wm;
+          if (bj * bj < rem) { a = min(a, j); e = max(e, j + 1); }
+        }
+      chord[i] = e > a ? (a | (e << 16)) : 0;
+    }
+    __syncthreads();
+    int best = 6, best_cost = -1, dummy0, dummy1;
+    {
+      const int lcs[3] = {3, 4, 6};
+      const int slots[3] = {MultiBand<NT, RS>::kSlots(3), MultiBand<NT, RS>::kSlots(4),
+                            MultiBand<NT, RS>::kSlots(6)};
+#pragma unroll
+      for (int ci = 0; ci < 3; ++ci) {
+        const int Lc = lcs[ci];
+        const int n = count_items(chord, hm, wm, Lc, -1, dummy0, dummy1);
+        const int nf = ((h + Lc - 1) / Lc) * w;
+        if (n > slots[ci] * NT || nf > slots[ci] * NT) continue;
+        const int cost = ((n + 63) / 64) * (27 * Lc + 28);
+        if (best_cost < 0 || cost < best_cost) { best = Lc; best_cost = cost; }
+      }
+    }
+    L = best;
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      const int q = (int)threadIdx.x + k * NT;
+      int b = -1, j = 0;
+      count_items(chord, hm, wm, L, q, b, j);
+      if (b < 0) continue;
+      const int i0 = b * L;
+      uint32_t m = 0u;
+      for (int l = 0; l < L; ++l) {
+        const int i = i0 + l;
+        if (i < hm && !masked(i, j, hm, wm, d4)) m |= 1u << l;
+      }
+      jac[k] = pack_item(j, i0, m);
+    }
+    const int nf = ((h + L - 1) / L) * w;
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      const int q = (int)threadIdx.x + k * NT;
+      if (q >= nf) continue;
+      const int band = q / w, j = 1 + (q - band * w), i0 = 1 + band * L;
+      uint32_t m = 0u;
+      for (int l = 0; l < L; ++l)
+        if (i0 + l <= h) m |= 1u << l;
+      fin[k] = pack_item(j, i0, m);
+    }
+    __syncthreads();
+  }
+
+  template <int LL>
+  __device__ void run_l(const DevParams &p, float *sub, int PV, const int roi[5], uint32_t &upd) {
+    constexpr int K = MultiBand<NT, RS>::kSlots(LL);
+    uint32_t j2[K], f2[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) { j2[k] = jac[k]; f2[k] = fin[k]; }
+    diffuse<NT, LL, K, RS, PS>(p, sub, PV, roi, j2, f2, upd);
+  }
+
+  __device__ void run(const DevParams &p, float *sub, int PV, const int roi[5], uint32_t &upd) {
+    if (L == 3) run_l<3>(p, sub, PV, roi, upd);
+    else if (L == 4) run_l<4>(p, sub, PV, roi, upd);
+    else run_l<6>(p, sub, PV, roi, upd);
+  }
+};
 
 // Banded in-place diffusion as a policy for the persistent kernel.
 template <int NT, int L, int KMAX, int RS, int PS>

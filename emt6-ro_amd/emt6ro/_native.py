@@ -7,7 +7,9 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libemt6ro_amd.so")
+# EMT6RO_LIB selects another build of the same C ABI (e.g. a debug build).
+LIB_PATH = os.environ.get("EMT6RO_LIB") or os.path.join(os.path.dirname(_HERE), "lib",
+                                                        "libemt6ro_amd.so")
 
 NUM_PARAMS = 40
 FIELDS = {

@@ -107,4 +107,6 @@ def fits(v, H, W):
         return False
     if v["kind"] == 0:
         return -(-H // v["l"]) * W <= v["kmax"] * v["nt"]
+    if v["kind"] == 2:
+        return True
     return H * W <= v["kmax"] * v["nt"] and (H - 2) * (W - 2) <= v["kf"] * v["nt"]
