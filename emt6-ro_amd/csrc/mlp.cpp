@@ -1,0 +1,376 @@
+// Fitness surrogate: an MLP in -> H -> H -> H -> 1 with GELU, BF16 weights and
+// activations on the matrix cores, FP32 accumulation, FP32 master weights and
+// Adam, trained by the kernels of mlp_kernels.hip.  C ABI at the end
+// (include/emt6ro_amd.h, emt6ro_mlp_*).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/emt6ro_amd.h"
+#include "host.h"
+#include "mlp.h"
+
+namespace emt6ro {
+
+#define EMT_HIPM(call)                                                                   \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      throw Error(cat("HIP error ", hipGetErrorString(e_), " at ", #call), -3);          \
+  } while (0)
+
+class Mlp {
+ public:
+  static constexpr int kLayers = 3;  // hidden layers
+
+  Mlp(int in, int hidden, uint64_t seed, int device)
+      : in_(in), kin_((in + 31) / 32 * 32), H_(hidden), device_(device) {
+    EMT_ENFORCE(in > 0 && hidden > 0 && hidden % 64 == 0, -1,
+                "MLP needs in > 0 and hidden a multiple of 64");
+    EMT_HIPM(hipSetDevice(device_));
+    EMT_HIPM(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    // Parameters: W1 [H, kin] (padded columns zero), b1, W2 [H, H], b2,
+    // W3 [H, H], b3, w4 [H], b4 [1].
+    sizes_ = {(int64_t)H_ * kin_, H_, (int64_t)H_ * H_, H_, (int64_t)H_ * H_, H_, H_, 1};
+    int64_t tot = 0;
+    for (int64_t n : sizes_) { offs_.push_back(tot); tot += n; }
+    total_ = tot;
+    w_ = dalloc<float>(tot); m_ = dalloc<float>(tot); v_ = dalloc<float>(tot); g_ = dalloc<float>(tot);
+    EMT_HIPM(hipMemsetAsync(m_, 0, tot * 4, s_));
+    EMT_HIPM(hipMemsetAsync(v_, 0, tot * 4, s_));
+    EMT_HIPM(hipMemsetAsync(g_, 0, tot * 4, s_));
+    for (int l = 0; l < kLayers; ++l) wb_[l] = dalloc<uint16_t>(sizes_[2 * l]);
+    // PyTorch-style init: U(-1/sqrt(fan_in), 1/sqrt(fan_in)).
+    std::mt19937_64 rng(seed);
+    std::vector<float> h(tot, 0.f);
+    const int fan[4] = {in_, H_, H_, H_};
+    for (int l = 0; l < 4; ++l) {
+      const float bnd = 1.f / std::sqrt((float)fan[l]);
+      std::uniform_real_distribution<float> u(-bnd, bnd);
+      float *W = &h[offs_[2 * l]], *b = &h[offs_[2 * l + 1]];
+      const int rows = l < 3 ? H_ : 1, cols = l == 0 ? in_ : H_, ld = l == 0 ? kin_ : H_;
+      for (int r = 0; r < rows; ++r)
+        for (int c = 0; c < cols; ++c) W[(int64_t)r * ld + c] = u(rng);
+      for (int r = 0; r < rows; ++r) b[r] = u(rng);
+    }
+    upload_params(h);
+  }
+
+  ~Mlp() {
+    (void)hipSetDevice(device_);
+    if (s_) (void)hipStreamSynchronize(s_);
+    for (void *p : allocs_) (void)hipFree(p);
+    if (s_) (void)hipStreamDestroy(s_);
+  }
+
+  int64_t num_params() const { return (int64_t)H_ * in_ + H_ + 2 * ((int64_t)H_ * H_ + H_) + H_ + 1; }
+
+  // Flat parameters without the padding columns: W1, b1, W2, b2, W3, b3, w4, b4.
+  void get_params(float *out) {
+    EMT_HIPM(hipSetDevice(device_));
+    std::vector<float> h(total_);
+    EMT_HIPM(hipMemcpyAsync(h.data(), w_, total_ * 4, hipMemcpyDeviceToHost, s_));
+    EMT_HIPM(hipStreamSynchronize(s_));
+    unpad(h, out);
+  }
+
+  void set_params(const float *in) {
+    EMT_HIPM(hipSetDevice(device_));
+    std::vector<float> h(total_, 0.f);
+    pad(in, h);
+    upload_params(h);
+    EMT_HIPM(hipMemsetAsync(m_, 0, total_ * 4, s_));
+    EMT_HIPM(hipMemsetAsync(v_, 0, total_ * 4, s_));
+    step_ = 0;
+  }
+
+  void set_data(const float *X, const float *t, int n) {
+    EMT_HIPM(hipSetDevice(device_));
+    EMT_ENFORCE(n > 0, -1, "empty data set");
+    release(data_x_);
+    release(data_t_);
+    n_data_ = 0;
+    std::vector<float> xp((size_t)n * kin_, 0.f);
+    for (int r = 0; r < n; ++r) std::memcpy(&xp[(size_t)r * kin_], &X[(size_t)r * in_], in_ * 4);
+    float *tmp = dalloc<float>(xp.size());
+    data_x_ = dalloc<uint16_t>(xp.size());
+    data_t_ = dalloc<float>(n);
+    EMT_HIPM(hipMemcpyAsync(tmp, xp.data(), xp.size() * 4, hipMemcpyHostToDevice, s_));
+    EMT_HIPM(mlp_to_bf16(tmp, data_x_, (int64_t)xp.size(), s_));
+    EMT_HIPM(hipMemcpyAsync(data_t_, t, (size_t)n * 4, hipMemcpyHostToDevice, s_));
+    EMT_HIPM(hipStreamSynchronize(s_));
+    release(tmp);
+    n_data_ = n;
+  }
+
+  // Adam on minibatches drawn from the data set on the device; returns each
+  // step's mean squared error.  The steps are enqueued back to back and the
+  // host waits once, at the end.
+  std::vector<float> train(int steps, int batch, float lr, uint64_t seed) {
+    EMT_HIPM(hipSetDevice(device_));
+    EMT_ENFORCE(n_data_ > 0, -1, "set_data first");
+    EMT_ENFORCE(steps >= 0 && batch > 0, -1, "steps >= 0 and batch > 0");
+    ensure_batch(batch);
+    std::vector<float> losses(steps);
+    if (steps == 0) return losses;
+    float *dl = dalloc<float>(steps);
+    for (int it = 0; it < steps; ++it) {
+      EMT_HIPM(mlp_gather(data_x_, data_t_, seed, (uint32_t)it, n_data_, x_, t_, batch, kin_, s_));
+      forward(batch);
+      backward(batch);
+      adam(lr);
+      EMT_HIPM(mlp_mean(sq_, batch, dl + it, s_));
+    }
+    EMT_HIPM(hipMemcpyAsync(losses.data(), dl, (size_t)steps * 4, hipMemcpyDeviceToHost, s_));
+    EMT_HIPM(hipStreamSynchronize(s_));
+    release(dl);
+    return losses;
+  }
+
+  void predict(const float *X, int n, float *y) {
+    EMT_HIPM(hipSetDevice(device_));
+    const int chunk = 4096;
+    for (int r0 = 0; r0 < n; r0 += chunk) {
+      const int b = std::min(chunk, n - r0);
+      load_batch(X + (size_t)r0 * in_, nullptr, b);
+      forward(b);
+      EMT_HIPM(hipMemcpyAsync(y + r0, y_, b * 4, hipMemcpyDeviceToHost, s_));
+    }
+    EMT_HIPM(hipStreamSynchronize(s_));
+  }
+
+  // Loss and gradient (flat, unpadded) of one batch, without an update.
+  float grads(const float *X, const float *t, int n, float *gout) {
+    EMT_HIPM(hipSetDevice(device_));
+    load_batch(X, t, n);
+    forward(n);
+    backward(n);
+    std::vector<float> h(total_), sq(n);
+    EMT_HIPM(hipMemcpyAsync(h.data(), g_, total_ * 4, hipMemcpyDeviceToHost, s_));
+    EMT_HIPM(hipMemcpyAsync(sq.data(), sq_, n * 4, hipMemcpyDeviceToHost, s_));
+    EMT_HIPM(hipStreamSynchronize(s_));
+    unpad(h, gout);
+    double s = 0;
+    for (float v : sq) s += v;
+    return (float)(s / (2.0 * n));
+  }
+
+ private:
+  template <typename T>
+  T *dalloc(size_t n) {
+    void *p = nullptr;
+    EMT_HIPM(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    allocs_.push_back(p);
+    return static_cast<T *>(p);
+  }
+  template <typename T>
+  void release(T *&p) {
+    if (!p) return;
+    auto it = std::find(allocs_.begin(), allocs_.end(), (void *)p);
+    if (it != allocs_.end()) allocs_.erase(it);
+    (void)hipFree(p);
+    p = nullptr;
+  }
+
+  void pad(const float *in, std::vector<float> &h) {
+    size_t o = 0;
+    for (int r = 0; r < H_; ++r)
+      for (int c = 0; c < in_; ++c) h[offs_[0] + (size_t)r * kin_ + c] = in[o++];
+    for (int i = 1; i < 8; ++i)
+      for (int64_t k = 0; k < sizes_[i]; ++k) h[offs_[i] + k] = in[o++];
+  }
+  void unpad(const std::vector<float> &h, float *out) {
+    size_t o = 0;
+    for (int r = 0; r < H_; ++r)
+      for (int c = 0; c < in_; ++c) out[o++] = h[offs_[0] + (size_t)r * kin_ + c];
+    for (int i = 1; i < 8; ++i)
+      for (int64_t k = 0; k < sizes_[i]; ++k) out[o++] = h[offs_[i] + k];
+  }
+
+  void upload_params(const std::vector<float> &h) {
+    EMT_HIPM(hipMemcpyAsync(w_, h.data(), total_ * 4, hipMemcpyHostToDevice, s_));
+    for (int l = 0; l < kLayers; ++l)
+      EMT_HIPM(mlp_to_bf16(w_ + offs_[2 * l], wb_[l], sizes_[2 * l], s_));
+    EMT_HIPM(hipStreamSynchronize(s_));
+  }
+
+  void ensure_batch(int B) {
+    if (B <= bmax_) return;
+    EMT_HIPM(hipStreamSynchronize(s_));
+    release(x_);
+    for (int l = 0; l < kLayers; ++l) { release(z_[l]); release(h_[l]); release(dz_[l]); }
+    release(y_); release(t_); release(dy_); release(sq_); release(ones_); release(xs_);
+    bmax_ = B;
+    x_ = dalloc<uint16_t>((size_t)B * kin_);
+    xs_ = dalloc<float>((size_t)B * kin_);
+    for (int l = 0; l < kLayers; ++l) {
+      z_[l] = dalloc<uint16_t>((size_t)B * H_);
+      h_[l] = dalloc<uint16_t>((size_t)B * H_);
+      dz_[l] = dalloc<uint16_t>((size_t)B * H_);
+    }
+    y_ = dalloc<float>(B); t_ = dalloc<float>(B); dy_ = dalloc<float>(B); sq_ = dalloc<float>(B);
+    ones_ = dalloc<uint16_t>(B);
+    std::vector<uint16_t> one(B, 0x3f80);  // bf16 1.0
+    EMT_HIPM(hipMemcpyAsync(ones_, one.data(), B * 2, hipMemcpyHostToDevice, s_));
+    EMT_HIPM(hipStreamSynchronize(s_));
+  }
+
+  // Host rows (in_ columns) -> padded bf16 input x_, targets -> t_.
+  void load_batch(const float *X, const float *t, int n) {
+    ensure_batch(n);
+    std::vector<float> xp((size_t)n * kin_, 0.f);
+    for (int r = 0; r < n; ++r) std::memcpy(&xp[(size_t)r * kin_], &X[(size_t)r * in_], in_ * 4);
+    EMT_HIPM(hipMemcpyAsync(xs_, xp.data(), xp.size() * 4, hipMemcpyHostToDevice, s_));
+    EMT_HIPM(mlp_to_bf16(xs_, x_, (int64_t)xp.size(), s_));
+    if (t) EMT_HIPM(hipMemcpyAsync(t_, t, n * 4, hipMemcpyHostToDevice, s_));
+    else EMT_HIPM(hipMemsetAsync(t_, 0, n * 4, s_));
+    EMT_HIPM(hipStreamSynchronize(s_));
+  }
+
+  void forward(int B) {
+    const uint16_t *in = x_;
+    int K = kin_;
+    for (int l = 0; l < kLayers; ++l) {
+      MlpGemm q{};
+      q.A = in; q.sam = K; q.sak = 1;
+      q.B = wb_[l]; q.sbk = 1; q.sbn = K;
+      q.M = B; q.N = H_; q.K = K;
+      q.epi = 0; q.bias = w_ + offs_[2 * l + 1]; q.act = 1;
+      q.z_out = z_[l]; q.h_out = h_[l]; q.ldc = H_;
+      EMT_HIPM(mlp_gemm(q, s_));
+      in = h_[l];
+      K = H_;
+    }
+    EMT_HIPM(mlp_gemv_out(h_[2], w_ + offs_[6], w_ + offs_[7], y_, B, H_, s_));
+    EMT_HIPM(mlp_loss(y_, t_, dy_, sq_, B, s_));
+  }
+
+  void backward(int B) {
+    // Output layer: dw4 = dy^T h3, db4 = sum dy, dz3 = dy w4 gelu'(z3).
+    EMT_HIPM(mlp_colsum(h_[2], dy_, g_ + offs_[6], B, H_, s_));
+    EMT_HIPM(mlp_colsum(ones_, dy_, g_ + offs_[7], B, 1, s_));
+    EMT_HIPM(mlp_out_back(dy_, w_ + offs_[6], z_[2], dz_[2], B, H_, s_));
+    for (int l = kLayers - 1; l >= 0; --l) {
+      const uint16_t *in = l == 0 ? x_ : h_[l - 1];
+      const int K = l == 0 ? kin_ : H_;
+      // dW_l = dz_l^T in  ([H, K], reduction over the batch)
+      MlpGemm q{};
+      q.A = dz_[l]; q.sam = 1; q.sak = H_;
+      q.B = in; q.sbk = K; q.sbn = 1;
+      q.M = H_; q.N = K; q.K = B;
+      q.epi = 2; q.f_out = g_ + offs_[2 * l]; q.ldc = K;
+      EMT_HIPM(mlp_gemm(q, s_));
+      EMT_HIPM(mlp_colsum(dz_[l], nullptr, g_ + offs_[2 * l + 1], B, H_, s_));
+      if (l > 0) {
+        // dz_{l-1} = (dz_l W_l) * gelu'(z_{l-1})
+        MlpGemm r{};
+        r.A = dz_[l]; r.sam = H_; r.sak = 1;
+        r.B = wb_[l]; r.sbk = H_; r.sbn = 1;
+        r.M = B; r.N = H_; r.K = H_;
+        r.epi = 1; r.z_in = z_[l - 1]; r.h_out = dz_[l - 1]; r.ldc = H_;
+        EMT_HIPM(mlp_gemm(r, s_));
+      }
+    }
+  }
+
+  void adam(float lr) {
+    ++step_;
+    for (int i = 0; i < 8; ++i) {
+      uint16_t *wb = (i % 2 == 0 && i < 6) ? wb_[i / 2] : nullptr;
+      EMT_HIPM(mlp_adam(w_ + offs_[i], m_ + offs_[i], v_ + offs_[i], g_ + offs_[i], wb,
+                        (int)sizes_[i], lr, 0.9f, 0.999f, 1e-8f, step_, s_));
+    }
+  }
+
+  int in_, kin_, H_, device_;
+  hipStream_t s_ = nullptr;
+  std::vector<int64_t> sizes_, offs_;
+  int64_t total_ = 0;
+  float *w_ = nullptr, *m_ = nullptr, *v_ = nullptr, *g_ = nullptr;
+  uint16_t *wb_[kLayers] = {};
+  int bmax_ = 0;
+  uint16_t *x_ = nullptr, *z_[kLayers] = {}, *h_[kLayers] = {}, *dz_[kLayers] = {}, *ones_ = nullptr;
+  float *y_ = nullptr, *t_ = nullptr, *dy_ = nullptr, *sq_ = nullptr, *xs_ = nullptr;
+  uint16_t *data_x_ = nullptr;
+  float *data_t_ = nullptr;
+  int n_data_ = 0;
+  int step_ = 0;
+  std::vector<void *> allocs_;
+};
+
+}  // namespace emt6ro
+
+struct emt6ro_mlp { std::unique_ptr<emt6ro::Mlp> m; };
+
+namespace {
+thread_local std::string g_mlp_error;
+template <typename F>
+int mlp_guarded(F &&f) {
+  try {
+    f();
+    return EMT6RO_OK;
+  } catch (const emt6ro::Error &e) {
+    g_mlp_error = e.what();
+    return e.code();
+  } catch (const std::exception &e) {
+    g_mlp_error = e.what();
+    return EMT6RO_ERR_ARG;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char *emt6ro_mlp_last_error(void) { return g_mlp_error.c_str(); }
+
+int emt6ro_mlp_create(int32_t in_features, int32_t hidden, uint64_t seed, int32_t device,
+                      emt6ro_mlp **out) {
+  return mlp_guarded([&] {
+    if (!out) throw emt6ro::Error("Check failed: out; null pointer", EMT6RO_ERR_ARG);
+    auto m = std::make_unique<emt6ro_mlp>();
+    m->m = std::make_unique<emt6ro::Mlp>(in_features, hidden, seed, device);
+    *out = m.release();
+  });
+}
+
+int64_t emt6ro_mlp_num_params(const emt6ro_mlp *m) { return m ? m->m->num_params() : -1; }
+
+int emt6ro_mlp_get_params(emt6ro_mlp *m, float *out) {
+  return mlp_guarded([&] { m->m->get_params(out); });
+}
+
+int emt6ro_mlp_set_params(emt6ro_mlp *m, const float *in) {
+  return mlp_guarded([&] { m->m->set_params(in); });
+}
+
+int emt6ro_mlp_set_data(emt6ro_mlp *m, const float *X, const float *y, int32_t n) {
+  return mlp_guarded([&] { m->m->set_data(X, y, n); });
+}
+
+int emt6ro_mlp_train(emt6ro_mlp *m, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                     float *losses) {
+  return mlp_guarded([&] {
+    std::vector<float> l = m->m->train(steps, batch, lr, seed);
+    if (losses) std::memcpy(losses, l.data(), l.size() * 4);
+  });
+}
+
+int emt6ro_mlp_predict(emt6ro_mlp *m, const float *X, int32_t n, float *y) {
+  return mlp_guarded([&] { m->m->predict(X, n, y); });
+}
+
+int emt6ro_mlp_grads(emt6ro_mlp *m, const float *X, const float *y, int32_t n, float *grads,
+                     float *loss) {
+  return mlp_guarded([&] {
+    const float l = m->m->grads(X, y, n, grads);
+    if (loss) *loss = l;
+  });
+}
+
+void emt6ro_mlp_destroy(emt6ro_mlp *m) { delete m; }
+
+}  // extern "C"

@@ -1,0 +1,297 @@
+// Fitness surrogate MLP kernels for gfx950: BF16 matrix-core GEMMs with FP32
+// accumulation and fused bias / GELU epilogues, GEMV kernels for the scalar
+// output layer, column reductions, and Adam.
+//
+// The GEMM computes C[m, n] = sum_k A(m, k) B(k, n) for A, B given by element
+// strides, so one kernel serves the forward product X W^T, the input gradient
+// dZ W and the weight gradient dZ^T X.  A 256-thread workgroup owns a 64x64
+// tile of C; each of its four waves owns a 32x32 quarter and accumulates it
+// with v_mfma_f32_32x32x8_bf16 over K in steps of 32 staged through LDS.
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "mlp.h"
+
+namespace emt6ro {
+namespace mlp {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// FP32 -> BF16, round to nearest even (NaN kept quiet).
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))).
+__device__ __forceinline__ float gelu(float x) {
+  const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
+  return 0.5f * x * (1.f + tanhf(u));
+}
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
+  const float t = tanhf(u);
+  return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * 0.7978845608028654f * (1.f + 0.134145f * x * x);
+}
+
+constexpr int kTile = 64, kTK = 32, kLdsK = kTK + 4;  // LDS row: 32 bf16 + pad
+
+// Epilogue operations.
+enum Epi : int {
+  kEpiForward = 0,   // z = acc + bias[n]; store z (bf16) and gelu(z) (bf16) or z if no act
+  kEpiBackGelu = 1,  // out = acc * gelu'(z[m, n]) (bf16)
+  kEpiStoreF32 = 2,  // out = acc (fp32)
+};
+
+struct GemmArgs {
+  const uint16_t *A, *B;
+  int64_t sam, sak, sbk, sbn;  // element strides
+  int M, N, K;
+  int epi;
+  const float *bias;           // kEpiForward
+  int act;                     // kEpiForward: apply GELU
+  uint16_t *z_out;             // kEpiForward: pre-activation (ldc)
+  uint16_t *h_out;             // kEpiForward / kEpiBackGelu output (ldc)
+  const uint16_t *z_in;        // kEpiBackGelu: pre-activation of this layer's input
+  float *f_out;                // kEpiStoreF32
+  int64_t ldc;
+};
+
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
+  __shared__ uint16_t As[kTile][kLdsK];
+  __shared__ uint16_t Bs[kTile][kLdsK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x16 acc;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+  const bool a_kfast = g.sak == 1, b_kfast = g.sbk == 1;
+  for (int k0 = 0; k0 < g.K; k0 += kTK) {
+    // Stage a 64 x 32 tile of A and of B (rows m / n, k contiguous in LDS).
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int idx = tid + e * 256;  // 0..2047
+      const int r = a_kfast ? idx / kTK : idx % kTile;
+      const int c = a_kfast ? idx % kTK : idx / kTile;
+      const int m = m0 + r, k = k0 + c;
+      As[r][c] = (m < g.M && k < g.K) ? g.A[m * g.sam + k * g.sak] : (uint16_t)0;
+      const int rb = b_kfast ? idx / kTK : idx % kTile;
+      const int cb = b_kfast ? idx % kTK : idx / kTile;
+      const int n = n0 + rb, kb = k0 + cb;
+      Bs[rb][cb] = (n < g.N && kb < g.K) ? g.B[kb * g.sbk + n * g.sbn] : (uint16_t)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kTK; kk += 8) {
+      const int kq = kk + (lane >> 5) * 4;
+      bf16x4 a, b;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[i] = (short)As[wm + (lane & 31)][kq + i];
+        b[i] = (short)Bs[wn + (lane & 31)][kq + i];
+      }
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // acc[j]: row 8 (j / 4) + 4 (lane / 32) + j % 4, column lane % 32.
+  const int n = n0 + wn + (lane & 31);
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int m = m0 + wm + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+    if (m >= g.M || n >= g.N) continue;
+    const int64_t o = (int64_t)m * g.ldc + n;
+    const float v = acc[j];
+    if (g.epi == kEpiForward) {
+      const float z = v + g.bias[n];
+      const uint16_t zb = to_bf16(z);
+      if (g.z_out) g.z_out[o] = zb;
+      g.h_out[o] = g.act ? to_bf16(gelu(from_bf16(zb))) : zb;
+    } else if (g.epi == kEpiBackGelu) {
+      g.h_out[o] = to_bf16(v * gelu_grad(from_bf16(g.z_in[o])));
+    } else {
+      g.f_out[o] = v;
+    }
+  }
+}
+
+// y[b] = sum_k h[b, k] w[k] + bias   (one wave per row)
+__global__ __launch_bounds__(256) void gemv_out_kernel(const uint16_t *h, const float *w,
+                                                       const float *bias, float *y, int B, int K) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= B) return;
+  float s = 0.f;
+  for (int k = lane; k < K; k += 64) s += from_bf16(h[(int64_t)row * K + k]) * w[k];
+  for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+  if (lane == 0) y[row] = s + bias[0];
+}
+
+// Loss gradient dy = (y - t) / B and per-row squared error.
+__global__ void loss_kernel(const float *y, const float *t, float *dy, float *sq, int B) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B) return;
+  const float d = y[i] - t[i];
+  dy[i] = d / (float)B;
+  sq[i] = d * d;
+}
+
+// dz[b, k] = dy[b] w[k] gelu'(z[b, k])  (bf16)
+__global__ void out_back_kernel(const float *dy, const float *w, const uint16_t *z, uint16_t *dz,
+                                int B, int K) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)B * K) return;
+  const int b = (int)(i / K), k = (int)(i % K);
+  dz[i] = to_bf16(dy[b] * w[k] * gelu_grad(from_bf16(z[i])));
+}
+
+// Column sums: out[k] = sum_b x[b, k] (x bf16) or sum_b dy[b] h[b, k].
+__global__ __launch_bounds__(256) void colsum_kernel(const uint16_t *x, const float *rowscale,
+                                                     float *out, int B, int K) {
+  __shared__ float part[4][64];
+  const int k = blockIdx.x * 64 + (threadIdx.x & 63), r0 = threadIdx.x >> 6;
+  float s = 0.f;
+  if (k < K)
+    for (int b = r0; b < B; b += 4) {
+      const float v = from_bf16(x[(int64_t)b * K + k]);
+      s += rowscale ? rowscale[b] * v : v;
+    }
+  part[r0][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (r0 == 0 && k < K) out[k] = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
+                                 part[3][threadIdx.x];
+}
+
+// Adam (Kingma & Ba, 2015) with bias correction; refreshes the bf16 copy.
+__global__ void adam_kernel(float *w, float *m, float *v, const float *g, uint16_t *wb, int n,
+                            float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  if (wb) wb[i] = to_bf16(wi);
+}
+
+__global__ void f32_to_bf16_kernel(const float *x, uint16_t *y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = to_bf16(x[i]);
+}
+
+// Minibatch row index: splitmix64 of (seed, step, row) reduced onto [0, n).
+__device__ __forceinline__ int32_t pick_row(uint64_t seed, uint32_t step, uint32_t b, int n) {
+  uint64_t z = seed + 0x9e3779b97f4a7c15ull * (((uint64_t)step << 32 | b) + 1);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  return (int32_t)(z % (uint64_t)n);
+}
+
+// Rows of a dataset (bf16, ld columns) gathered into a minibatch; the rows
+// are drawn on the device, so a training step needs no host round trip.
+__global__ void gather_kernel(const uint16_t *X, const float *t, uint64_t seed, uint32_t step,
+                              int n, uint16_t *Xb, float *tb, int B, int ld) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)B * ld) return;
+  const int b = (int)(i / ld), c = (int)(i % ld);
+  const int r = pick_row(seed, step, (uint32_t)b, n);
+  Xb[i] = X[(int64_t)r * ld + c];
+  if (c == 0) tb[b] = t[r];
+}
+
+// out[0] = mean of sq[0..B) (one workgroup).
+__global__ __launch_bounds__(256) void mean_kernel(const float *sq, int B, float *out) {
+  __shared__ float part[256];
+  float s = 0.f;
+  for (int i = threadIdx.x; i < B; i += 256) s += sq[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = part[0] / (float)B;
+}
+
+}  // namespace mlp
+
+// ---------------------------------------------------------------------------
+// Launchers.
+// ---------------------------------------------------------------------------
+namespace {
+inline int blocks(int64_t n, int t) { return (int)((n + t - 1) / t); }
+}  // namespace
+
+hipError_t mlp_gemm(const MlpGemm &q, hipStream_t s) {
+  mlp::GemmArgs g{};
+  g.A = q.A; g.B = q.B;
+  g.sam = q.sam; g.sak = q.sak; g.sbk = q.sbk; g.sbn = q.sbn;
+  g.M = q.M; g.N = q.N; g.K = q.K;
+  g.epi = q.epi; g.bias = q.bias; g.act = q.act;
+  g.z_out = q.z_out; g.h_out = q.h_out; g.z_in = q.z_in; g.f_out = q.f_out; g.ldc = q.ldc;
+  dim3 grid((q.N + mlp::kTile - 1) / mlp::kTile, (q.M + mlp::kTile - 1) / mlp::kTile);
+  hipLaunchKernelGGL(mlp::gemm_bf16_kernel, grid, dim3(256), 0, s, g);
+  return hipGetLastError();
+}
+
+hipError_t mlp_gemv_out(const uint16_t *h, const float *w, const float *bias, float *y, int B,
+                        int K, hipStream_t s) {
+  hipLaunchKernelGGL(mlp::gemv_out_kernel, dim3(blocks(B, 4)), dim3(256), 0, s, h, w, bias, y, B, K);
+  return hipGetLastError();
+}
+
+hipError_t mlp_loss(const float *y, const float *t, float *dy, float *sq, int B, hipStream_t s) {
+  hipLaunchKernelGGL(mlp::loss_kernel, dim3(blocks(B, 256)), dim3(256), 0, s, y, t, dy, sq, B);
+  return hipGetLastError();
+}
+
+hipError_t mlp_out_back(const float *dy, const float *w, const uint16_t *z, uint16_t *dz, int B,
+                        int K, hipStream_t s) {
+  hipLaunchKernelGGL(mlp::out_back_kernel, dim3(blocks((int64_t)B * K, 256)), dim3(256), 0, s, dy, w,
+                     z, dz, B, K);
+  return hipGetLastError();
+}
+
+hipError_t mlp_colsum(const uint16_t *x, const float *rowscale, float *out, int B, int K,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(mlp::colsum_kernel, dim3(blocks(K, 64)), dim3(256), 0, s, x, rowscale, out, B, K);
+  return hipGetLastError();
+}
+
+hipError_t mlp_adam(float *w, float *m, float *v, const float *g, uint16_t *wb, int n, float lr,
+                    float b1, float b2, float eps, int t, hipStream_t s) {
+  float c1 = 1.f, c2 = 1.f;
+  for (int i = 0; i < t; ++i) { c1 *= b1; c2 *= b2; }
+  c1 = 1.f - c1;
+  c2 = 1.f - c2;
+  hipLaunchKernelGGL(mlp::adam_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, w, m, v, g, wb, n, lr,
+                     b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t mlp_to_bf16(const float *x, uint16_t *y, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(mlp::f32_to_bf16_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, x, y, n);
+  return hipGetLastError();
+}
+
+hipError_t mlp_gather(const uint16_t *X, const float *t, uint64_t seed, uint32_t step, int n,
+                      uint16_t *Xb, float *tb, int B, int ld, hipStream_t s) {
+  hipLaunchKernelGGL(mlp::gather_kernel, dim3(blocks((int64_t)B * ld, 256)), dim3(256), 0, s, X, t,
+                     seed, step, n, Xb, tb, B, ld);
+  return hipGetLastError();
+}
+
+hipError_t mlp_mean(const float *sq, int B, float *out, hipStream_t s) {
+  hipLaunchKernelGGL(mlp::mean_kernel, dim3(1), dim3(256), 0, s, sq, B, out);
+  return hipGetLastError();
+}
+
+}  // namespace emt6ro

@@ -72,6 +72,16 @@ def _load():
         "emt6ro_instr_read": ([C.POINTER(u64), C.POINTER(C.c_double)], C.c_int),
         "emt6ro_instr_reset": ([], C.c_int),
         "emt6ro_instr_histograms": ([C.POINTER(u64), i32], C.c_int),
+        "emt6ro_mlp_last_error": ([], C.c_char_p),
+        "emt6ro_mlp_create": ([i32, i32, u64, i32, C.POINTER(P)], C.c_int),
+        "emt6ro_mlp_num_params": ([P], C.c_int64),
+        "emt6ro_mlp_get_params": ([P, f32p], C.c_int),
+        "emt6ro_mlp_set_params": ([P, f32p], C.c_int),
+        "emt6ro_mlp_set_data": ([P, f32p, f32p, i32], C.c_int),
+        "emt6ro_mlp_train": ([P, i32, i32, C.c_float, u64, f32p], C.c_int),
+        "emt6ro_mlp_predict": ([P, f32p, i32, f32p], C.c_int),
+        "emt6ro_mlp_grads": ([P, f32p, f32p, i32, f32p, f32p], C.c_int),
+        "emt6ro_mlp_destroy": ([P], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -88,4 +98,11 @@ def check(rc):
     reference's std::runtime_error the same way)."""
     if rc != 0:
         raise RuntimeError(lib.emt6ro_last_error().decode())
+    return rc
+
+
+def check_mlp(rc):
+    """`check` for the emt6ro_mlp_* calls, which keep their own last error."""
+    if rc != 0:
+        raise RuntimeError(lib.emt6ro_mlp_last_error().decode())
     return rc

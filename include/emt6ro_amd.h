@@ -176,6 +176,32 @@ int emt6ro_batch_set_doses(emt6ro_batch *b, const float *prot_doses);
  * up to hmax x wmax (0 if it does not fit). */
 int emt6ro_variant_occupancy(int32_t variant, int32_t hmax, int32_t wmax);
 
+/* ---- Fitness surrogate: MLP in -> hidden x3 (GELU) -> 1 on the matrix cores
+ * (no reference counterpart: the reference's only surrogate is an external
+ * pretrained Temporal Fusion Transformer, python_ga/example_run_model.py:
+ * 24-184).  BF16 weights and activations, FP32 accumulation, FP32 master
+ * weights and Adam.  Parameters are flat FP32 in the order W1 [hidden x in],
+ * b1, W2, b2, W3, b3, w4 [hidden], b4 [1]. ----------------------------------- */
+typedef struct emt6ro_mlp emt6ro_mlp;
+const char *emt6ro_mlp_last_error(void);
+/* hidden must be a multiple of 64. */
+int emt6ro_mlp_create(int32_t in_features, int32_t hidden, uint64_t seed, int32_t device,
+                      emt6ro_mlp **out);
+int64_t emt6ro_mlp_num_params(const emt6ro_mlp *m);
+int emt6ro_mlp_get_params(emt6ro_mlp *m, float *out);
+int emt6ro_mlp_set_params(emt6ro_mlp *m, const float *in);
+/* Training set: X [n x in_features] (row-major), targets y [n]. */
+int emt6ro_mlp_set_data(emt6ro_mlp *m, const float *X, const float *y, int32_t n);
+/* `steps` Adam steps on random minibatches of `batch` rows; losses[steps]
+ * (may be NULL) receives each step's mean squared error. */
+int emt6ro_mlp_train(emt6ro_mlp *m, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                     float *losses);
+int emt6ro_mlp_predict(emt6ro_mlp *m, const float *X, int32_t n, float *y);
+/* Loss (mean squared error / 2) and its gradient for one batch, no update. */
+int emt6ro_mlp_grads(emt6ro_mlp *m, const float *X, const float *y, int32_t n, float *grads,
+                     float *loss);
+void emt6ro_mlp_destroy(emt6ro_mlp *m);
+
 /* ---- Instrumentation (common/instr.h:33-58; backend.cc:221-231) -------- */
 
 /* The reference compiles its device counters only with EMT6RO_INSTRUMENT

@@ -1,0 +1,157 @@
+"""The fitness-surrogate MLP on the GPU (emt6-ro_amd/csrc/mlp_kernels.hip,
+mlp.cpp) against the NumPy oracle (oracle/mlp_oracle.py): forward values,
+gradients, one Adam step, and training.
+
+Tolerances: against the oracle's BF16-rounding mode the GPU differs only by
+FP32 accumulation order and the BF16 roundings that order flips, so the
+relative L2 error of outputs and of each gradient tensor is held to 1e-2;
+against the plain FP64 network the BF16 storage itself (8 significant bits)
+is the error, held to 5e-2 relative L2 and a gradient cosine above 0.995."""
+import sys
+
+import numpy as np
+import pytest
+
+import mlp_oracle as M
+
+pytestmark = pytest.mark.gpu
+
+N_IN, HID = 152, 512
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-30))
+
+
+def split(flat, n_in, hidden):
+    return M.unflatten(flat, n_in, hidden)
+
+
+@pytest.fixture(scope="module")
+def S():
+    import emt6ro.surrogate as S
+    return S
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((n, N_IN)).astype(np.float32)
+    t = (0.5 * X[:, 0] + 0.3 * X[:, 1] * X[:, 2] + np.sin(X[:, 3]) +
+         0.2 * X[:, :8].sum(axis=1)).astype(np.float32)
+    return X, t
+
+
+def test_params_round_trip_and_count(S):
+    log("[surrogate] params")
+    net = S.MLP(N_IN, HID, seed=1)
+    assert net.num_params == 604161
+    p = net.get_params()
+    W1, b1, W2, b2, W3, b3, w4, b4 = split(p, N_IN, HID)
+    assert np.abs(W1).max() <= 1 / np.sqrt(N_IN) and np.abs(W2).max() <= 1 / np.sqrt(HID)
+    q = M.init_params(N_IN, HID, seed=9).astype(np.float32)
+    net.set_params(q)
+    np.testing.assert_array_equal(net.get_params(), q)
+    with pytest.raises(ValueError):
+        net.set_params(q[:-1])
+    with pytest.raises(RuntimeError, match="multiple of 64"):
+        S.MLP(N_IN, 100)
+
+
+@pytest.mark.parametrize("n", [1, 300, 4096 + 17])
+def test_forward_matches_oracle(S, n):
+    log(f"[surrogate] forward n={n}")
+    net = S.MLP(N_IN, HID, seed=2)
+    p = net.get_params().astype(np.float64)
+    X, _ = _data(n, n)
+    y = net.predict(X)
+    y16, _ = M.forward(p, X.astype(np.float64), N_IN, HID, bf16_mode=True)
+    y64, _ = M.forward(p, X.astype(np.float64), N_IN, HID)
+    e16, e64 = rel(y, y16), rel(y, y64)
+    log(f"  rel L2 vs bf16 oracle {e16:.2e}, vs fp64 {e64:.2e}")
+    assert e16 < 1e-2
+    assert e64 < 5e-2
+
+
+def test_gradients_match_oracle(S):
+    log("[surrogate] gradients")
+    net = S.MLP(N_IN, HID, seed=3)
+    p = net.get_params().astype(np.float64)
+    X, t = _data(700, 7)
+    loss, g = net.grads(X, t)
+    l16, g16 = M.backward(p, X.astype(np.float64), t.astype(np.float64), N_IN, HID,
+                          bf16_mode=True)
+    l64, g64 = M.backward(p, X.astype(np.float64), t.astype(np.float64), N_IN, HID)
+    names = ["W1", "b1", "W2", "b2", "W3", "b3", "w4", "b4"]
+    for name, a, b, c in zip(names, split(g, N_IN, HID), split(g16, N_IN, HID),
+                             split(g64, N_IN, HID)):
+        e16, e64 = rel(a, b), rel(a, c)
+        cos = float(np.dot(a.ravel(), c.ravel()) /
+                    max(np.linalg.norm(a) * np.linalg.norm(c), 1e-30))
+        log(f"  {name}: rel L2 vs bf16 oracle {e16:.2e}, vs fp64 {e64:.2e}, cos {cos:.5f}")
+        assert e16 < 1e-2, name
+        assert e64 < 5e-2 and cos > 0.995, name
+    assert abs(loss - l16) / l16 < 1e-3
+    assert abs(loss - l64) / l64 < 2e-2
+
+
+def test_one_adam_step_moves_each_weight_by_lr_against_its_gradient(S):
+    """With a data set of one repeated row every minibatch is that row, so the
+    first Adam step is w - lr * g / (|g| + eps) (bias-corrected moments equal
+    g and g^2): each weight with a clear gradient moves by -lr sign(g)."""
+    log("[surrogate] adam step")
+    net = S.MLP(N_IN, HID, seed=4)
+    p0 = net.get_params().astype(np.float64)
+    X, t = _data(1, 11)
+    Xr = np.repeat(X, 64, axis=0)
+    tr = np.repeat(t, 64)
+    _, g16 = M.backward(p0, X.astype(np.float64), t.astype(np.float64), N_IN, HID,
+                        bf16_mode=True)
+    net.set_data(Xr, tr)
+    lr = 1e-3
+    losses = net.train(1, batch=32, lr=lr, seed=5)
+    y0, _ = M.forward(p0, X.astype(np.float64), N_IN, HID, bf16_mode=True)
+    assert abs(losses[0] - (y0[0] - t[0]) ** 2) <= 1e-2 * (y0[0] - t[0]) ** 2
+    d = net.get_params().astype(np.float64) - p0
+    big = np.abs(g16) > 1e-3 * np.abs(g16).max()
+    log(f"  {int(big.sum())} weights with |g| > 1e-3 max|g|")
+    assert big.sum() > 1000
+    np.testing.assert_allclose(d[big], -lr * np.sign(g16[big]), rtol=2e-3, atol=2e-7)
+    assert np.abs(d).max() <= lr * (1 + 1e-3)
+
+
+def test_training_reduces_loss_and_generalises(S):
+    log("[surrogate] training")
+    X, t = _data(20000, 13)
+    Xh, th = _data(2000, 14)
+    net = S.MLP(N_IN, HID, seed=6)
+    net.set_data(X, t)
+    losses = net.train(600, batch=1024, lr=1e-3, seed=1)
+    first, last = losses[:20].mean(), losses[-20:].mean()
+    yh = net.predict(Xh)
+    mae = float(np.abs(yh - th).mean())
+    base = float(np.abs(th - t.mean()).mean())
+    log(f"  loss {first:.4f} -> {last:.4f}; held-out MAE {mae:.4f} (predict-the-mean {base:.4f})")
+    assert np.isfinite(losses).all()
+    # An FP32 PyTorch network of the same shape, trained the same way, ends
+    # near loss 0.002 and held-out MAE 0.1 (predict-the-mean 1.0).
+    assert last < 0.05 * first
+    assert mae < 0.25 * base
+
+
+def test_surrogate_standardises_and_errors(S):
+    log("[surrogate] wrapper")
+    rng = np.random.default_rng(0)
+    X = rng.random((3000, S.N_FEATURES)).astype(np.float32) * 2.0
+    y = 500.0 + 100.0 * X[:, 0] - 50.0 * X[:, 150]
+    s = S.Surrogate(seed=1)
+    losses = s.fit(X, y, steps=400, batch=512, lr=1e-3, seed=2)
+    assert losses[-10:].mean() < losses[:10].mean()
+    pred = s.predict(X[:500])
+    assert np.abs(pred - y[:500]).mean() < 0.1 * np.abs(y[:500] - y.mean()).mean() + 5.0
+    with pytest.raises(ValueError):
+        s.net.predict(np.zeros((2, 3), np.float32))
