@@ -315,8 +315,9 @@ def mutate_time_value(rng, population, config):
     out = []
     for g in population:
         g = np.array(g, float)
-        for idx in range(len(g)):
-            if rng.uniform() < prob and g[idx] > 0:
+        # One uniform per gene, drawn up front; genes are then visited in order.
+        for idx in np.nonzero(rng.random(len(g)) < prob)[0]:
+            if g[idx] > 0:
                 t = int(rng.integers(len(g)))
                 v = g[t] + g[idx]
                 rem = 0.0
@@ -334,10 +335,9 @@ def mutate_swap(rng, population, config):
     out = []
     for g in population:
         g = np.array(g, float)
-        for idx in range(len(g)):
-            if rng.uniform() < prob:
-                k = (idx + int(rng.integers(len(g) - 1))) % len(g)
-                g[idx], g[k] = g[k], g[idx]
+        for idx in np.nonzero(rng.random(len(g)) < prob)[0]:
+            k = (idx + int(rng.integers(len(g) - 1))) % len(g)
+            g[idx], g[k] = g[k], g[idx]
         out.append(refine_to_time_constraint(rng, g, interval, config))
     return out
 

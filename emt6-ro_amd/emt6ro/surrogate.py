@@ -30,7 +30,7 @@ def _fp(a):
 
 def tumour_descriptor(state):
     """The 8 descriptors of a TumorState (float32)."""
-    occ = state.occupancy == 1
+    occ = state.occupancy() == 1
     n = int(occ.sum())
     out = np.zeros(N_DESCRIPTORS, np.float32)
     out[0] = n
@@ -39,12 +39,12 @@ def tumour_descriptor(state):
     rr, cc = np.nonzero(occ)
     r0, c0 = rr.mean(), cc.mean()
     out[1] = np.sqrt(((rr - r0) ** 2 + (cc - c0) ** 2).max())
-    ph = state.phase[occ]
+    ph = state.phase()[occ]
     for k in range(4):
         out[2 + k] = np.count_nonzero(ph == k) / n
     out[5] += np.count_nonzero(ph == 4) / n
-    out[6] = state.ox[occ].mean()
-    out[7] = state.cho[occ].mean()
+    out[6] = state.ox()[occ].mean()
+    out[7] = state.cho()[occ].mean()
     return out
 
 
@@ -155,8 +155,23 @@ class Surrogate:
         sd = X.std(axis=0)
         self.x_sd = np.where(sd > 1e-6, sd, 1.0).astype(np.float32)
         self.y_mu, self.y_sd = float(y.mean()), float(max(y.std(), 1e-6))
+        self._base = (X, y)
         self.net.set_data(self._nx(X), ((y - self.y_mu) / self.y_sd).astype(np.float32))
         return self.net.train(steps, batch=min(batch, len(y)), lr=lr, seed=seed)
+
+    def refit(self, X, y, steps, batch=1024, lr=3e-4, seed=1):
+        """Fine-tune on new samples mixed with the original training set (the
+        new rows repeated to make up at least a quarter of the mix); the
+        standardisation of the first fit is kept."""
+        X = np.asarray(X, np.float32)
+        y = np.asarray(y, np.float64)
+        bx, by = self._base
+        rep = max(1, int(np.ceil(len(by) / (3.0 * len(y)))))
+        Xa = np.concatenate([bx, np.tile(X, (rep, 1))])
+        ya = np.concatenate([by, np.tile(y, rep)])
+        self.net.set_data(self._nx(Xa), ((ya - self.y_mu) / self.y_sd).astype(np.float32))
+        self._refits = getattr(self, "_refits", 0) + 1
+        return self.net.train(steps, batch=min(batch, len(ya)), lr=lr, seed=seed + self._refits)
 
     def predict(self, X):
         return self.net.predict(self._nx(X)).astype(np.float64) * self.y_sd + self.y_mu
