@@ -167,13 +167,13 @@ struct CellL {
   uint32_t mode, phase;
 };
 
-// calcDelayTime (cell.cu:241-248)
+// calcDelayTime (cell.cu:93-100)
 __device__ __forceinline__ float delay_time(float R, const DevParams &p) {
   return p.delay_coeff * expf_fixed(p.delay_exp * R);
 }
 
 // updateState, enterG1SStopping, tryProliferating, tryQuiescence
-// (cell.cu:181-230); returns alive and sets mode.
+// (cell.cu:33-82); returns alive and sets mode.
 __device__ __forceinline__ bool update_state(CellL &c, float cho, float ox, float gi,
                                              const DevParams &p, int vacant) {
   if (c.prolif >= c.t4 - p.death_margin) return false;
@@ -213,7 +213,7 @@ __device__ __forceinline__ bool progress_clock(CellL &c, const DevParams &p) {
   return false;
 }
 
-// irradiate (cell.cu:232-239)
+// irradiate (cell.cu:84-91)
 __device__ __forceinline__ void irradiate(CellL &c, float dose, const DevParams &p) {
   c.R = c.R / (1.f + c.t_rep / p.repair_half_time) + dose;
   c.t_rep = 0.f;

@@ -228,8 +228,8 @@ HostParams HostParams::from_values(const float *v) {
 }
 
 // Derived constants, in the reference's float expressions: clock step
-// time_step / 3600 (cell.cu:173, cell.h:107), G1/S margin 2*time_step / 3600.f
-// (cell.cu:216), diffusion coefficient ((D * dt) * HS) / f with HS = (float)
+// time_step / 3600 (cell.cu:25, cell.h:107), G1/S margin 2*time_step / 3600.f
+// (cell.cu:68), diffusion coefficient ((D * dt) * HS) / f with HS = (float)
 // (2 sqrt2) and f = (float)(4 + HS) (diffusion.cu:119-120,163), substeps
 // (int16)(time_step / dt) (simulation.cu:170, diffusion.h).
 DevParams HostParams::device() const {

@@ -54,8 +54,8 @@ enum : uint32_t { kPurposeRepair = 1, kPurposeNeighbour = 2, kPurposeNormal = 3 
 struct DevParams {
   float time_step;          // seconds per step
   float hour_step;          // time_step / 3600 (clock and repair increments)
-  float death_margin;       // time_step / 3600 (updateState, cell.cu:222)
-  float g1s_margin;         // 2 * time_step / 3600 (enterG1SStopping, cell.cu:216)
+  float death_margin;       // time_step / 3600 (updateState, cell.cu:74)
+  float g1s_margin;         // 2 * time_step / 3600 (enterG1SStopping, cell.cu:68)
   float ext[3];             // external substrate levels
   float k[3];               // ((D * dt_diff) * HS) / F per substrate (diffusion.cu:163)
   int32_t substeps;         // (int16)(time_step / dt_diff)

@@ -177,7 +177,7 @@ void eo_normals5(uint32_t seed, uint32_t epoch, uint32_t sim, uint32_t step,
 /* Cell rules                                                               */
 /* ------------------------------------------------------------------------ */
 
-/* cell/cell.cu:181-230 (updateState, enterG1SStopping, tryProliferating,
+/* cell/cell.cu:33-82 (updateState, enterG1SStopping, tryProliferating,
  * tryQuiescence).  Returns 1 if the cell stays alive; sets mode. */
 int eo_update_state(eo_cell *c, const float sub[3], const eo_params *p, int vacant) {
   if (c->prolif >= c->times[4] - p->time_step / 3600.f) return 0;
@@ -226,12 +226,12 @@ void eo_metabolise(const eo_cell *c, float sub[3], const eo_params *p) {
   sub[2] = sub[2] - p->metab[c->mode][2];
 }
 
-/* cell/cell.cu:241-248 (calcDelayTime) */
+/* cell/cell.cu:93-100 (calcDelayTime) */
 float eo_delay_time(float R, const eo_params *p) {
   return p->delay_coeff * eo_expf(p->delay_exp * R);
 }
 
-/* cell/cell.cu:232-239 */
+/* cell/cell.cu:84-91 (irradiate) */
 void eo_irradiate(eo_cell *c, float dose, const eo_params *p) {
   c->R = c->R / (1.f + c->t_rep / p->repair_half_time) + dose;
   c->t_rep = 0.f;
@@ -253,7 +253,7 @@ int eo_try_repair(eo_cell *c, int cycle_changed, const eo_params *p, float u) {
   return 1;
 }
 
-/* simulation/cell-division.h:382-404 (chooseNeighbour): diagonal neighbours
+/* simulation/cell-division.h:44-57 (chooseNeighbour): diagonal neighbours
  * get weight 1 each, orthogonal ones sqrt(2) each.  Returns the direction
  * number (0-3 diagonal, 4-7 orthogonal) and the child coordinates. */
 int eo_choose_neighbour(int r, int c, float u, int *cr, int *cc) {
