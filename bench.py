@@ -45,6 +45,10 @@ def parse():
     ap.add_argument("--sims", type=int, default=0,
                     help="use only the first N simulations (profiling runs)")
     ap.add_argument("--tune-steps", type=int, default=600)
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the surrogate training / screening sample")
+    ap.add_argument("--surrogate-only", action="store_true",
+                    help="only run the surrogate sample (profiling runs)")
     ap.add_argument("--json-out", default=None)
     return ap.parse_args()
 
@@ -90,6 +94,37 @@ def cpu_baseline(params, wl, n_sims, hours):
                       f"full {hours} h, CPU oracle (C, FP32, OpenMP over simulations)"}
 
 
+def surrogate_sample(train_steps=200, batch=4096, n_data=200000, n_screen=100000):
+    """The fitness surrogate (152-512-512-512-1, BF16 on the matrix cores):
+    Adam steps on random data and one screening pass, timed on the host."""
+    import numpy as np
+    from emt6ro.surrogate import MLP, N_FEATURES
+    hidden = 512
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((n_data, N_FEATURES)).astype(np.float32)
+    y = rng.standard_normal(n_data).astype(np.float32)
+    net = MLP(N_FEATURES, hidden, seed=0)
+    net.set_data(X, y)
+    net.train(10, batch=batch, lr=1e-4, seed=1)            # warm-up
+    t0 = time.perf_counter()
+    net.train(train_steps, batch=batch, lr=1e-4, seed=2)
+    train_s = time.perf_counter() - t0
+    Xs = rng.standard_normal((n_screen, N_FEATURES)).astype(np.float32)
+    net.predict(Xs[:4096])
+    t0 = time.perf_counter()
+    net.predict(Xs)
+    screen_s = time.perf_counter() - t0
+    # GEMM FLOP per sample: padded input 160 columns (K padded to 32).
+    kin = (N_FEATURES + 31) // 32 * 32
+    fwd = 2 * (kin * hidden + 2 * hidden * hidden) * batch
+    # forward, weight gradients (same shapes), input gradients of layers 2-3
+    step_flop = 2 * fwd + 2 * (2 * hidden * hidden) * batch
+    return {"train_steps_per_s": train_steps / train_s,
+            "train_tflops_mfma_gemm": step_flop * train_steps / train_s / 1e12,
+            "batch": batch, "screen_rows_per_s": n_screen / screen_s,
+            "note": "host-timed; includes the host<->device copies of predict"}
+
+
 def copy_calibration(gib=1):
     """A plain device copy of known size, to scale rocprofv3's FETCH_SIZE and
     WRITE_SIZE counters (run under the profiler with --copy-calibrate)."""
@@ -107,6 +142,9 @@ def main():
     args = parse()
     if args.copy_calibrate:
         copy_calibration()
+        return
+    if args.surrogate_only:
+        print(json.dumps({"surrogate": surrogate_sample()}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the simulations are "
@@ -192,6 +230,14 @@ def main():
         pers1 = n / (time.perf_counter() - t0)
         out["persistent_first_hour"] = {"value": pers1, "unit": "tumour-hours/s"}
         out["speedup_vs_plain_first_hour"] = pers1 / plain
+    if not args.no_surrogate:
+        sur = surrogate_sample()
+        out["surrogate"] = sur
+        # One GA fitness evaluation = 3 tumours x 20 replicates x 144 h.
+        plain_evals = value / (3 * 20 * 144)
+        out["ga_evals_per_s"] = {"plain_simulator": plain_evals,
+                                 "surrogate_screen": sur["screen_rows_per_s"] / 3,
+                                 "ratio": sur["screen_rows_per_s"] / 3 / plain_evals}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(params, wl, args.cpu_sims, 144)
     line = json.dumps(out)

@@ -65,6 +65,12 @@ const char *emt6ro_last_error(void) { return g_last_error.c_str(); }
 
 const char *emt6ro_version(void) { return "emt6ro_amd 0.1.0 (gfx950)"; }
 
+int emt6ro_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
 int emt6ro_params_load(const char *path, emt6ro_params **out) {
   return guarded([&] {
     need(path && out, "null pointer");
@@ -436,18 +442,34 @@ int emt6ro_variant_occupancy(int32_t variant, int32_t hmax, int32_t wmax) {
 int emt6ro_instr_read(uint64_t counters[4], double *sum_death_prob) {
   return guarded([&] {
     need(counters && sum_death_prob, "null pointer");
-    for (int i = 0; i < 4; ++i) counters[i] = 0;
-    *sum_death_prob = 0.0;
+    std::vector<unsigned long long> rec(emt6ro::dev::kInstrN);
+    emt6ro::instr_collect(rec.data(), false);
+    for (int i = 0; i < 4; ++i) counters[i] = rec[i];
+    *sum_death_prob = (double)rec[emt6ro::dev::kInstrDeathProbFx] / 4294967296.0;
   });
 }
 
-int emt6ro_instr_reset(void) { return EMT6RO_OK; }
+int emt6ro_instr_reset(void) {
+  return guarded([&] {
+    std::vector<unsigned long long> rec(emt6ro::dev::kInstrN);
+    emt6ro::instr_collect(rec.data(), true);
+  });
+}
 
 int emt6ro_instr_histograms(uint64_t *dst, int32_t n) {
   return guarded([&] {
     need(dst != nullptr || n == 0, "null pointer");
-    for (int32_t i = 0; i < n; ++i) dst[i] = 0;
+    std::vector<unsigned long long> rec(emt6ro::dev::kInstrN);
+    emt6ro::instr_collect(rec.data(), false);
+    for (int32_t i = 0; i < n; ++i)
+      dst[i] = i < 2 * emt6ro::dev::kInstrHistN ? rec[emt6ro::dev::kInstrIrrHist + i] : 0;
   });
 }
+
+int emt6ro_instr_enable(int32_t on) {
+  return guarded([&] { emt6ro::instr_set_enabled(on != 0); });
+}
+
+int emt6ro_instr_enabled(void) { return emt6ro::instr_enabled() ? 1 : 0; }
 
 }  // extern "C"

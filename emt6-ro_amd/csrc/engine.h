@@ -22,6 +22,16 @@ namespace emt6ro {
 
 enum RunPath { kPathPersistent = 0, kPathStep = 1, kPathStepGraph = 2 };
 
+// Instrumentation counters (dev::kInstr* layout), one record per device,
+// allocated on first use.  Runs started while instrumentation is enabled add
+// to the record of their device.  Enabled at load time when the environment
+// variable EMT6RO_INSTRUMENT is set to anything but "" or "0".
+bool instr_enabled();
+void instr_set_enabled(bool on);
+unsigned long long *instr_device_record(int device);  // null while disabled
+// Sum of every device's record (kInstrN values); zeroes them when `reset`.
+void instr_collect(unsigned long long *out, bool reset);
+
 struct BatchSpec {
   std::vector<int32_t> tumor_index, prot_index;
   std::vector<uint32_t> horizon, sim_id;

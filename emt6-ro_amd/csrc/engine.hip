@@ -4,7 +4,11 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numeric>
 
 namespace emt6ro {
@@ -244,6 +248,60 @@ void Batch::ensure_scratch(int wgs) {
   scratch_wgs_ = wgs;
 }
 
+// ---------------------------------------------------------------------------
+// Instrumentation records.
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_instr_mu;
+std::map<int, unsigned long long *> g_instr_rec;
+std::atomic<int> g_instr_on{-1};  // -1: not read from the environment yet
+}  // namespace
+
+bool instr_enabled() {
+  int v = g_instr_on.load();
+  if (v < 0) {
+    const char *e = std::getenv("EMT6RO_INSTRUMENT");
+    v = (e && *e && std::strcmp(e, "0") != 0) ? 1 : 0;
+    int expect = -1;
+    g_instr_on.compare_exchange_strong(expect, v);
+    v = g_instr_on.load();
+  }
+  return v == 1;
+}
+
+void instr_set_enabled(bool on) { g_instr_on.store(on ? 1 : 0); }
+
+unsigned long long *instr_device_record(int device) {
+  if (!instr_enabled()) return nullptr;
+  std::lock_guard<std::mutex> lk(g_instr_mu);
+  auto it = g_instr_rec.find(device);
+  if (it != g_instr_rec.end()) return it->second;
+  void *p = nullptr;
+  EMT_HIP(hipSetDevice(device));
+  EMT_HIP(hipMalloc(&p, sizeof(unsigned long long) * dev::kInstrN));
+  EMT_HIP(hipMemset(p, 0, sizeof(unsigned long long) * dev::kInstrN));
+  EMT_HIP(hipDeviceSynchronize());
+  g_instr_rec[device] = static_cast<unsigned long long *>(p);
+  return g_instr_rec[device];
+}
+
+void instr_collect(unsigned long long *out, bool reset) {
+  std::lock_guard<std::mutex> lk(g_instr_mu);
+  for (int i = 0; i < dev::kInstrN; ++i) out[i] = 0;
+  std::vector<unsigned long long> h(dev::kInstrN);
+  for (auto &kv : g_instr_rec) {
+    EMT_HIP(hipSetDevice(kv.first));
+    EMT_HIP(hipDeviceSynchronize());
+    EMT_HIP(hipMemcpy(h.data(), kv.second, sizeof(unsigned long long) * dev::kInstrN,
+                      hipMemcpyDeviceToHost));
+    for (int i = 0; i < dev::kInstrN; ++i) out[i] += h[i];
+    if (reset) {
+      EMT_HIP(hipMemset(kv.second, 0, sizeof(unsigned long long) * dev::kInstrN));
+      EMT_HIP(hipDeviceSynchronize());
+    }
+  }
+}
+
 dev::RunArgs Batch::args(uint32_t nsteps) const {
   dev::RunArgs a{};
   a.p = dparams_;
@@ -267,6 +325,7 @@ dev::RunArgs Batch::args(uint32_t nsteps) const {
   a.ovf = ovf_;
   a.ovf_stride = capmax_;
   a.pv_max = 0;
+  a.instr = instr_device_record(device_);
   return a;
 }
 

@@ -108,7 +108,8 @@ __global__ __launch_bounds__(NT, WPE) void persistent_kernel(RunArgs a) {
       }
       diff.run(a.p, sub, PV, roi, updates);
       const float dose = dose_at(a, d.prot, t);
-      cells_step<NT>(a.p, key, t, dose, SubLds<RS>{sub, PV}, st, H, W, slots, d.cap, ovf, sm);
+      cells_step<NT>(a.p, key, t, dose, SubLds<RS>{sub, PV}, st, H, W, slots, d.cap, ovf, sm,
+                     a.instr);
       if ((k & 1023u) == 1023u) { updates64 += updates; updates = 0u; }
     }
     updates64 += updates;
@@ -237,7 +238,7 @@ __global__ __launch_bounds__(NT) void cells_kernel(RunArgs a) {
   const RngKey key{a.seed, a.epoch, d.sim_id};
   const float dose = dose_at(a, d.prot, t);
   cells_step<NT>(a.p, key, t, dose, SubGlobal{a.sub_pool + d.sub_off, W, HW}, st, H, W, slots,
-                 d.cap, ovf, sm);
+                 d.cap, ovf, sm, a.instr);
   for (int k = threadIdx.x; k < (HW + 3) / 4; k += NT) gs[k] = ls[k];
   if (threadIdx.x == 0) {
     hd->step = t + 1u;

@@ -49,7 +49,31 @@ struct RunArgs {
   uint64_t *ovf;              // per-workgroup ready-entry spill, ovf_stride entries
   int64_t ovf_stride;
   int32_t pv_max;             // LDS floats per substrate plane
+  unsigned long long *instr;  // instrumentation counters (null: off), kInstr* layout
 };
+
+// Instrumentation record (common/instr.h:5-21): repair decisions, repair
+// kills, irradiation events, divisions, the sum of death probabilities in
+// 32.32 fixed point (an integer sum, so the total does not depend on the
+// order of the atomic adds), then 300-bin histograms of the irradiation and
+// of the time in repair at each repair decision (0.1 per bin).
+enum : int {
+  kInstrRepairDecisions = 0,
+  kInstrRepairKills = 1,
+  kInstrIrradiationEvents = 2,
+  kInstrDivisions = 3,
+  kInstrDeathProbFx = 4,
+  kInstrHistN = 300,
+  kInstrIrrHist = 5,
+  kInstrTirHist = 5 + 300,
+  kInstrN = 5 + 600,
+};
+
+// Histogram bin of common/instr.h:42-44 (cell.h:115-120).
+__device__ __forceinline__ int instr_bin(float v) {
+  int b = (int)(v / 0.1f);
+  return b < 0 ? 0 : (b >= kInstrHistN ? kInstrHistN - 1 : b);
+}
 
 // Protocol::getDose (protocol.h:9-14)
 __device__ __forceinline__ float dose_at(const RunArgs &a, int prot, uint32_t t) {
@@ -433,20 +457,21 @@ __device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5]
       for (int s = 0; s < 3; ++s) {
         const float *P = sub + s * PV + a0;
         const float kk = p.k[s];
-        float ul = P[-1], uc = P[0], ur = P[1];
-        float ml = P[RS - 1], mc = P[RS], mr = P[RS + 1];
+        // Rolling window down the column: centre values of the rows above,
+        // at and below, and their horizontal neighbour sums hu, hm, hd.
+        float uc = P[0], hu = P[-1] + P[1];
+        float mc = P[RS], hm = P[RS - 1] + P[RS + 1];
 #pragma unroll
         for (int l = 0; l < L; ++l) {
           const float *dn = P + (l + 2) * RS;
-          const float dl = dn[-1], dc = dn[0], dr = dn[1];
-          const float vl = ul + dl, vc = uc + dc, vr = ur + dr;
-          const float o = vc + (ml + mr);
-          const float dd = vl + vr;
+          const float dc = dn[0], hd = dn[-1] + dn[1];
+          const float o = (uc + dc) + hm;
+          const float dd = hu + hd;
           float Lp = fmaf(dd, R2, o);
           Lp = fmaf(-F, mc, Lp);
           res[k][l][s] = fmaf(kk, Lp, mc);
-          ul = ml; uc = mc; ur = mr;
-          ml = dl; mc = dc; mr = dr;
+          uc = mc; hu = hm;
+          mc = dc; hm = hd;
         }
       }
     }
@@ -481,7 +506,7 @@ __device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5]
 // Banded in-place diffusion whose band height is chosen per ROI.  For each
 // candidate height the plan counts the band items and picks the height with
 // the fewest wave instructions, estimated as (wave rounds of 64 items) x
-// (27 L + 28) VALU operations per item; items are packed into waves in
+// (21 L + 22) VALU operations per item; items are packed into waves in
 // order (item q on thread q mod NT, round q / NT) so unused waves leave their
 // SIMDs to other workgroups.
 template <int NT, int RS>
@@ -547,7 +572,7 @@ struct MultiBandDiffuser {
         const int n = count_items(chord, hm, wm, Lc, -1, dummy0, dummy1);
         const int nf = ((h + Lc - 1) / Lc) * w;
         if (n > slots[ci] * NT || nf > slots[ci] * NT) continue;
-        const int cost = ((n + 63) / 64) * (27 * Lc + 28);
+        const int cost = ((n + 63) / 64) * (21 * Lc + 22);
         if (best_cost < 0 || cost < best_cost) { best = Lc; best_cost = cost; }
       }
     }
@@ -680,12 +705,12 @@ __device__ __forceinline__ void pp_substep(const DevParams &p, const float *src,
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
       const float *P = src + s * PV + site[k];
-      const float ul = P[-RS - 1], uc = P[-RS], ur = P[-RS + 1];
-      const float ml = P[-1], mc = P[0], mr = P[1];
-      const float dl = P[RS - 1], dc = P[RS], dr = P[RS + 1];
-      const float vl = ul + dl, vc = uc + dc, vr = ur + dr;
-      const float o = vc + (ml + mr);
-      const float dd = vl + vr;
+      const float mc = P[0];
+      const float hu = P[-RS - 1] + P[-RS + 1];
+      const float hm = P[-1] + P[1];
+      const float hd = P[RS - 1] + P[RS + 1];
+      const float o = (P[-RS] + P[RS]) + hm;
+      const float dd = hu + hd;
       float Lp = fmaf(dd, R2, o);
       Lp = fmaf(-F, mc, Lp);
       dst[s * PV + site[k]] = fmaf(p.k[s], Lp, mc);
@@ -825,7 +850,8 @@ struct SubGlobal {
 template <int NT, typename SubRef>
 __device__ void cells_step(const DevParams &p, const RngKey &key, uint32_t t, float dose,
                            const SubRef &sr, uint8_t *st, int H, int W, uint32_t *slots,
-                           int cap, uint64_t *ovf, Shared &sm) {
+                           int cap, uint64_t *ovf, Shared &sm,
+                           unsigned long long *instr = nullptr) {
   const int tid = threadIdx.x;
   if (tid == 0) { sm.ready_cnt = 0; sm.winner = kNoWinner; }
   __syncthreads();
@@ -864,6 +890,7 @@ __device__ void cells_step(const DevParams &p, const RngKey &key, uint32_t t, fl
         irradiate(cl, dose, p);
         delay = cl.delay;
         delay_loaded = true;
+        if (instr) atomicAdd(&instr[kInstrIrradiationEvents], 1ull);
       }
       cho = cho - p.metab[cl.mode][0];
       ox = ox - p.metab[cl.mode][1];
@@ -878,6 +905,14 @@ __device__ void cells_step(const DevParams &p, const RngKey &key, uint32_t t, fl
         if (cl.t_rep >= delay) {
           const float death_prob = 1.f - p.surv_coeff * expf_fixed(p.surv_exp * cl.R);
           const float u = uniform(key, t, (uint32_t)x, kPurposeRepair);
+          if (instr) {
+            atomicAdd(&instr[kInstrRepairDecisions], 1ull);
+            atomicAdd(&instr[kInstrDeathProbFx],
+                      (unsigned long long)((double)death_prob * 4294967296.0));
+            atomicAdd(&instr[kInstrIrrHist + instr_bin(cl.R)], 1ull);
+            atomicAdd(&instr[kInstrTirHist + instr_bin(cl.t_rep)], 1ull);
+            if (u < death_prob) atomicAdd(&instr[kInstrRepairKills], 1ull);
+          }
           if (u < death_prob) {
             alive = false;
           } else {
@@ -969,6 +1004,7 @@ __device__ void cells_step(const DevParams &p, const RngKey &key, uint32_t t, fl
       st[child] = kOccupied;
       sm.n = nn + 1;
       sm.divisions += 1u;
+      if (instr) atomicAdd(&instr[kInstrDivisions], 1ull);
     } else {
       sm.status |= 1u;
     }

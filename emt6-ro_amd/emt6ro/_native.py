@@ -33,6 +33,7 @@ def _load():
     sig = {
         "emt6ro_last_error": ([], C.c_char_p),
         "emt6ro_version": ([], C.c_char_p),
+        "emt6ro_device_count": ([], C.c_int),
         "emt6ro_params_load": ([C.c_char_p, C.POINTER(P)], C.c_int),
         "emt6ro_params_from_values": ([f32p, C.c_int, C.POINTER(P)], C.c_int),
         "emt6ro_params_values": ([P, f32p, C.c_int], C.c_int),
@@ -72,6 +73,8 @@ def _load():
         "emt6ro_instr_read": ([C.POINTER(u64), C.POINTER(C.c_double)], C.c_int),
         "emt6ro_instr_reset": ([], C.c_int),
         "emt6ro_instr_histograms": ([C.POINTER(u64), i32], C.c_int),
+        "emt6ro_instr_enable": ([i32], C.c_int),
+        "emt6ro_instr_enabled": ([], C.c_int),
         "emt6ro_mlp_last_error": ([], C.c_char_p),
         "emt6ro_mlp_create": ([i32, i32, u64, i32, C.POINTER(P)], C.c_int),
         "emt6ro_mlp_num_params": ([P], C.c_int64),

@@ -36,6 +36,9 @@ typedef struct emt6ro_batch emt6ro_batch;
 const char *emt6ro_last_error(void);
 /* Library version string. */
 const char *emt6ro_version(void);
+/* Number of visible GPUs (0 when none or no driver); no reference
+ * counterpart. */
+int emt6ro_device_count(void);
 
 /* ---- Parameters (parameters/parameters.h:10-72) ------------------------- */
 
@@ -204,12 +207,22 @@ void emt6ro_mlp_destroy(emt6ro_mlp *m);
 
 /* ---- Instrumentation (common/instr.h:33-58; backend.cc:221-231) -------- */
 
-/* The reference compiles its device counters only with EMT6RO_INSTRUMENT
- * and reads zeros otherwise; this build keeps per-simulation division
- * counts (emt6ro_batch_counters) and reports the global counters as zero. */
+/* counters = repair decisions, repair kills, irradiation events, divisions
+ * (InstrCounters, common/instr.h:46-52); histograms: 300 bins of the
+ * irradiation then 300 of the time in repair at each repair decision.
+ * Summed over the devices used; integer totals (the death-probability sum is
+ * kept in 32.32 fixed point), so they do not depend on thread timing.
+ * Counters read zero while instrumentation is off, as the reference's do in
+ * its default build. */
 int emt6ro_instr_read(uint64_t counters[4], double *sum_death_prob);
 int emt6ro_instr_reset(void);
 int emt6ro_instr_histograms(uint64_t *dst, int32_t n);
+/* The reference compiles the counters in with -DEMT6RO_INSTRUMENT
+ * (common/instr.h:18-21); here runs started while instrumentation is enabled
+ * count.  Default: on when the environment variable EMT6RO_INSTRUMENT is set
+ * to anything but "" or "0". */
+int emt6ro_instr_enable(int32_t on);
+int emt6ro_instr_enabled(void);
 
 #ifdef __cplusplus
 }

@@ -238,19 +238,43 @@ void eo_irradiate(eo_cell *c, float dose, const eo_params *p) {
   c->delay = eo_delay_time(c->R, p);
 }
 
-/* cell/cell.h:102-136 with the uniform supplied by the caller. */
-int eo_try_repair(eo_cell *c, int cycle_changed, const eo_params *p, float u) {
+/* Histogram bin of the instrumentation (common/instr.h:42-44, cell.h:115-120):
+ * value / 0.1f truncated, clamped to [0, 299]. */
+static int eo_instr_bin(float v) {
+  int b = (int)(v / 0.1f);
+  if (b < 0) b = 0;
+  if (b >= EO_INSTR_HIST_N) b = EO_INSTR_HIST_N - 1;
+  return b;
+}
+
+/* cell/cell.h:102-136 with the uniform supplied by the caller; `instr` (may
+ * be NULL) receives the counters of cell.h:111-126. */
+static int eo_try_repair_instr(eo_cell *c, int cycle_changed, const eo_params *p, float u,
+                               uint64_t *instr) {
   if (c->t_rep > 0 || (c->R > 0 && cycle_changed)) {
     c->t_rep = c->t_rep + p->time_step / 3600.f;
     if (c->t_rep >= c->delay) {
       float death_prob = 1.f - p->surv_coeff * eo_expf(p->surv_exp * c->R);
-      if (u < death_prob) return 0;
+      if (instr) {
+        instr[EO_INSTR_REPAIR_DECISIONS] += 1;
+        instr[EO_INSTR_DEATH_PROB_FX] += (uint64_t)((double)death_prob * 4294967296.0);
+        instr[EO_INSTR_IRR_HIST + eo_instr_bin(c->R)] += 1;
+        instr[EO_INSTR_TIR_HIST + eo_instr_bin(c->t_rep)] += 1;
+      }
+      if (u < death_prob) {
+        if (instr) instr[EO_INSTR_REPAIR_KILLS] += 1;
+        return 0;
+      }
       c->R = 0.f;
       c->t_rep = 0.f;
       c->delay = 0.f;
     }
   }
   return 1;
+}
+
+int eo_try_repair(eo_cell *c, int cycle_changed, const eo_params *p, float u) {
+  return eo_try_repair_instr(c, cycle_changed, p, u, NULL);
 }
 
 /* simulation/cell-division.h:44-57 (chooseNeighbour): diagonal neighbours
@@ -396,20 +420,23 @@ void eo_diffuse_f64(const eo_params *p, int H, int W, const double *sub_in,
   free(act);
 }
 
-/* FP32 stencil in the build's fixed order:
- *   V(c) = up(c) + down(c);  o = V(c) + (left + right);  d = V(c-1) + V(c+1)
+/* FP32 stencil in the build's fixed order, with H(r) = left(r) + right(r)
+ * the sum of a row's two horizontal neighbours of the centre column:
+ *   o = (up + down) + H(mid);  d = H(up row) + H(down row)
  *   L = fma(d, 1/sqrt2, o);  L = fma(-F, C, L);  C' = fma(k, L, C)
+ * (a column walking down the rows computes each H once and reuses it as the
+ * diagonal sum of the rows above and below)
  * with F = (float)(4 + (float)(2 sqrt2)) and k = ((D dt) HS) / F in float, the
  * coefficient expression of diffusion.cu:163. */
 static float eo_stencil_f32(const float *t, int i, int tw, float k) {
   const float R2 = (float)0.70710678118654752440;
   const float HS = (float)(2.0 * 1.41421356237309504880);
   const float F = (float)(4.0 + (double)HS);
-  float vc = t[i - tw] + t[i + tw];
-  float vl = t[i - tw - 1] + t[i + tw - 1];
-  float vr = t[i - tw + 1] + t[i + tw + 1];
-  float o = vc + (t[i - 1] + t[i + 1]);
-  float d = vl + vr;
+  float hu = t[i - tw - 1] + t[i - tw + 1];
+  float hm = t[i - 1] + t[i + 1];
+  float hd = t[i + tw - 1] + t[i + tw + 1];
+  float o = (t[i - tw] + t[i + tw]) + hm;
+  float d = hu + hd;
   float L = fmaf(d, R2, o);
   L = fmaf(-F, t[i], L);
   return fmaf(k, L, t[i]);
@@ -534,7 +561,7 @@ uint32_t eo_arbitration_key(uint32_t i) {
 static int eo_cells_step(const eo_params *p, int H, int W, float *sub, uint8_t *st,
                          eo_cell *cells, uint16_t *list, int32_t *pn, int cap,
                          int32_t *owner, uint8_t *flags, uint32_t t, uint32_t sim,
-                         float dose, uint32_t seed, uint32_t epoch) {
+                         float dose, uint32_t seed, uint32_t epoch, uint64_t *instr) {
   const int HW = H * W;
   const int n = *pn;
   for (int i = 0; i < n; ++i) {
@@ -550,13 +577,16 @@ static int eo_cells_step(const eo_params *p, int H, int W, float *sub, uint8_t *
     int ready = 0;
     int alive = eo_update_state(c, s3, p, flags[i] & 0x0f);
     if (alive) {
-      if (dose > 0) eo_irradiate(c, dose, p);
+      if (dose > 0) {
+        eo_irradiate(c, dose, p);
+        if (instr) instr[EO_INSTR_IRRADIATION_EVENTS] += 1;
+      }
       eo_metabolise(c, s3, p);
       int changed = eo_progress_clock(c, p->time_step);
       int need_u = (c->t_rep > 0 || (c->R > 0 && changed)) &&
                    (c->t_rep + p->time_step / 3600.f >= c->delay);
       float u = need_u ? eo_uniform(seed, epoch, sim, t, (uint32_t)x, EO_PURPOSE_REPAIR) : 1.f;
-      alive = eo_try_repair(c, changed, p, u);
+      alive = eo_try_repair_instr(c, changed, p, u, instr);
       if (alive) ready = (c->phase == 4);
     }
     sub[x] = s3[0];
@@ -603,6 +633,7 @@ static int eo_cells_step(const eo_params *p, int H, int W, float *sub, uint8_t *
     list[n] = (uint16_t)child;
     owner[child] = n;
     *pn = n + 1;
+    if (instr) instr[EO_INSTR_DIVISIONS] += 1;
   }
   return 0;
 }
@@ -611,7 +642,7 @@ static int eo_run_one(const eo_params *p, float *sub, uint8_t *st, eo_cell *cell
                       uint16_t *list, int32_t *pn, int32_t cap, int32_t *roi,
                       uint32_t *pstep, uint32_t sim, const float *dose_row,
                       int32_t n_slots, uint32_t res, uint32_t length, uint32_t seed,
-                      uint32_t epoch, uint32_t nsteps) {
+                      uint32_t epoch, uint32_t nsteps, uint64_t *instr) {
   const int H = p->H, W = p->W, HW = H * W;
   int32_t *owner = (int32_t *)malloc(sizeof(int32_t) * HW);
   uint8_t *flags = (uint8_t *)malloc((size_t)cap + 1);
@@ -630,7 +661,7 @@ static int eo_run_one(const eo_params *p, float *sub, uint8_t *st, eo_cell *cell
     float dose = 0.f;
     if (t < length && t % res == 0 && (int32_t)(t / res) < n_slots) dose = dose_row[t / res];
     if (eo_cells_step(p, H, W, sub, st, cells, list, pn, cap, owner, flags, t, sim, dose,
-                      seed, epoch) != 0) {
+                      seed, epoch, instr) != 0) {
       rc = -1;
       break;
     }
@@ -646,6 +677,17 @@ int eo_run(const eo_params *p, int B, float *sub, uint8_t *st, eo_cell *cells,
            const int32_t *sim_id, const int32_t *prot_index, const float *prot_data,
            int32_t n_slots, uint32_t prot_res, uint32_t prot_length,
            uint32_t seed, uint32_t epoch, uint32_t nsteps, int nthreads) {
+  return eo_run_instr(p, B, sub, st, cells, list, nlist, cap, roi, step, sim_id, prot_index,
+                      prot_data, n_slots, prot_res, prot_length, seed, epoch, nsteps, nthreads,
+                      NULL);
+}
+
+int eo_run_instr(const eo_params *p, int B, float *sub, uint8_t *st, eo_cell *cells,
+                 uint16_t *list, int32_t *nlist, int32_t cap, int32_t *roi, uint32_t *step,
+                 const int32_t *sim_id, const int32_t *prot_index, const float *prot_data,
+                 int32_t n_slots, uint32_t prot_res, uint32_t prot_length,
+                 uint32_t seed, uint32_t epoch, uint32_t nsteps, int nthreads,
+                 uint64_t *instr) {
   const size_t HW = (size_t)p->H * p->W;
   int err = 0;
 #ifdef _OPENMP
@@ -656,7 +698,8 @@ int eo_run(const eo_params *p, int B, float *sub, uint8_t *st, eo_cell *cells,
     int rc = eo_run_one(p, sub + (size_t)b * 3 * HW, st + (size_t)b * HW, cells + (size_t)b * HW,
                         list + (size_t)b * cap, nlist + b, cap, roi + 5 * b, step + b,
                         (uint32_t)sim_id[b], prot_data + (size_t)prot_index[b] * n_slots,
-                        n_slots, prot_res, prot_length, seed, epoch, nsteps);
+                        n_slots, prot_res, prot_length, seed, epoch, nsteps,
+                        instr ? instr + (size_t)b * EO_INSTR_N : NULL);
     if (rc != 0) err |= 1;
   }
   (void)nthreads;

@@ -111,6 +111,29 @@ int eo_run(const eo_params *p, int B, float *sub, uint8_t *st, eo_cell *cells,
            int32_t n_slots, uint32_t prot_res, uint32_t prot_length,
            uint32_t seed, uint32_t epoch, uint32_t nsteps, int nthreads);
 
+/* eo_run with the instrumentation counters of common/instr.h:5-21 per
+ * simulation: instr (may be NULL) holds B records of EO_INSTR_N uint64:
+ * repair decisions, repair kills, irradiation events, divisions, the sum of
+ * death probabilities in 32.32 fixed point, then the 300-bin histograms of
+ * irradiation and of time in repair (0.1 per bin) at each repair decision. */
+enum {
+  EO_INSTR_REPAIR_DECISIONS = 0,
+  EO_INSTR_REPAIR_KILLS = 1,
+  EO_INSTR_IRRADIATION_EVENTS = 2,
+  EO_INSTR_DIVISIONS = 3,
+  EO_INSTR_DEATH_PROB_FX = 4,
+  EO_INSTR_HIST_N = 300,
+  EO_INSTR_IRR_HIST = 5,
+  EO_INSTR_TIR_HIST = 5 + 300,
+  EO_INSTR_N = 5 + 600
+};
+int eo_run_instr(const eo_params *p, int B, float *sub, uint8_t *st, eo_cell *cells,
+                 uint16_t *list, int32_t *nlist, int32_t cap, int32_t *roi, uint32_t *step,
+                 const int32_t *sim_id, const int32_t *prot_index, const float *prot_data,
+                 int32_t n_slots, uint32_t prot_res, uint32_t prot_length,
+                 uint32_t seed, uint32_t epoch, uint32_t nsteps, int nthreads,
+                 uint64_t *instr);
+
 uint32_t eo_count_living(int H, int W, const uint8_t *st);
 
 #ifdef __cplusplus

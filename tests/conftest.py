@@ -25,8 +25,11 @@ def default_params_path():
 
 
 def gpu_available():
+    """Asks the library's own HIP runtime.  (PyTorch is not used here: its
+    bundled HIP runtime and the library's must not both initialise in one
+    process.)"""
     try:
-        import torch
-        return torch.cuda.is_available()
+        from emt6ro._native import lib
+        return lib.emt6ro_device_count() > 0
     except Exception:  # noqa: BLE001
         return False

@@ -3,12 +3,15 @@ rounding against torch's conversion, the FP64 network against torch's GELU
 and autograd, and the analytic gradient against finite differences."""
 import numpy as np
 import pytest
-import torch
 
 import mlp_oracle as M
 
+# PyTorch is imported inside the tests: a GPU run deselects them, and its
+# HIP runtime must not load beside the library's in that process.
+
 
 def test_bf16_matches_torch_round_to_nearest_even():
+    import torch
     rng = np.random.default_rng(0)
     x = np.concatenate([
         rng.standard_normal(20000).astype(np.float32) * 10.0 ** rng.integers(-8, 8, 20000),
@@ -23,6 +26,7 @@ def test_bf16_matches_torch_round_to_nearest_even():
 
 
 def test_gelu_closed_forms():
+    import torch
     assert M.gelu(np.array(0.0)) == 0.0
     x = np.array([-30.0, 30.0])
     np.testing.assert_allclose(M.gelu(x), [0.0, 30.0], atol=1e-12)
@@ -35,6 +39,7 @@ def test_gelu_closed_forms():
 
 
 def _torch_loss_and_grad(flat, X, t, n_in, hidden):
+    import torch
     p = torch.tensor(flat, dtype=torch.float64, requires_grad=True)
     parts, o = [], 0
     for s in [(hidden, n_in), (hidden,), (hidden, hidden), (hidden,), (hidden, hidden),

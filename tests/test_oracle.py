@@ -499,3 +499,32 @@ def test_oracle_run_is_deterministic_and_threads_agree():
         runs.append((b.counts(), b.sub.copy(), b.st.copy()))
     assert (runs[0][0] == runs[1][0]).all()
     assert np.array_equal(runs[0][1], runs[1][1]) and np.array_equal(runs[0][2], runs[1][2])
+
+
+def test_oracle_instrumentation_counters_are_consistent():
+    """The oracle's instrumentation (common/instr.h:5-21): counting leaves the
+    simulation unchanged; kills <= decisions; each histogram holds one entry
+    per decision; the mean death probability lies in [0, 1]; no dose, no
+    irradiation events; divisions equal the list growth between rebuilds."""
+    import oracle as O
+    from conftest import golden
+    v = O.read_params_json(golden("default-parameters.json"))
+    s = O.read_state_text(golden("tumor-lib", "tumor-3.txt"), v)
+    n0 = int((s["st"] == O.OCCUPIED).sum())
+    doses = np.zeros((2, 480), np.float32)
+    doses[1, [0, 2]] = [3.0, 2.0]
+    a = O.OracleBatch(v, [s, s], doses, [0, 1], [0, 1], 300, 144000)
+    b = O.OracleBatch(v, [s, s], doses, [0, 1], [0, 1], 300, 144000)
+    a.run(127, seed=9, instr=True)
+    b.run(127, seed=9)
+    np.testing.assert_array_equal(a.st, b.st)
+    np.testing.assert_array_equal(a.sub, b.sub)
+    r0, r1 = a.instr[0], a.instr[1]
+    assert r0[2] == 0 and r0[0] == 0          # no dose: no irradiation, no repair
+    assert int(r0[3]) == int(a.nlist[0]) - n0   # one list append per division
+    assert 0 < r1[2] <= 2 * n0
+    a.run(5000, seed=9, instr=True)   # repair delay 3.34 exp(0.149 R) h
+    r1 = a.instr[1]
+    assert 0 < r1[1] <= r1[0]
+    assert r1[5:305].sum() == r1[0] == r1[305:].sum()
+    assert 0.0 <= r1[4] / 2 ** 32 / r1[0] <= 1.0
