@@ -26,7 +26,7 @@ def test_instr_counters_match_oracle():
     params = sim.load_parameters(golden("default-parameters.json"))
     v = params.values()
     st, arr = disk_state(params, 53, 53, radius=10, seed=3)
-    n, steps = 6, 4800   # past the repair delay 3.34 exp(0.149 R) h of the doses below
+    n, steps = 6, 9000   # repair starts at the next cycle change, ends 3.34 exp(0.149 R) h later
     doses = np.zeros((2, 480), np.float32)
     doses[0, [0, 4]] = [2.5, 2.0]
     doses[1, [1, 6]] = [4.0, 1.0]
@@ -53,7 +53,7 @@ def test_instr_counters_match_oracle():
     np.testing.assert_array_equal(got, base)  # counting does not change results
     orc = O.OracleBatch(v, [oracle_state_from_arrays(arr, v)] * n, doses, prot_index,
                         np.arange(n), 300, 144000)
-    orc.run(steps, seed=5, instr=True)
+    orc.run(steps, seed=5, nthreads=6, instr=True)
     np.testing.assert_array_equal(got, orc.counts())
     rec = orc.instr.sum(axis=0)
     log(f"  decisions {c.repair_decisions} kills {c.repair_kills} irradiation "

@@ -27,7 +27,7 @@ def test_replay_matches_experiment_and_writes_rows(tmp_path):
     pf = tmp_path / "prot.txt"
     P.write_two_line(prots, pf)
     canon = P.canonical(P.parse_two_line(pf))
-    steps, reps, seed = 1800, 4, 77
+    steps, reps, seed = 3600, 4, 77
     log("[replay] chunked replay vs one Experiment")
     got = R.replay(params, tumors, canon, reps, steps, 300, seed, max_sims=2 * 3 * reps)
     e = sim.Experiment(params, tumors, reps, len(canon), simulation_steps=steps,
