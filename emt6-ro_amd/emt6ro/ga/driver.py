@@ -117,6 +117,15 @@ def run_plain_ga(rng, fitness, config, pop_size=100, generations=50, n_slots=144
     return history
 
 
+def _spearman(a, b):
+    """Rank correlation of two samples (ties broken by order)."""
+    ra = np.argsort(np.argsort(a)).astype(np.float64)
+    rb = np.argsort(np.argsort(b)).astype(np.float64)
+    if ra.std() == 0 or rb.std() == 0:
+        return 0.0
+    return float(np.corrcoef(ra, rb)[0, 1])
+
+
 def surrogate_fitness(model, genomes, descriptors):
     """Predicted fitness: 1500 minus the predicted mean count over tumours."""
     g = np.asarray(genomes, np.float32)
@@ -165,6 +174,7 @@ def run_surrogate_ga(rng, fitness, model, descriptors, config, pop_size=100, gen
             vfit = fitness(chosen)
         evals += len(chosen)
         err.append(float(np.mean(np.abs(pred[top] - vfit))))
+        rank_corr = _spearman(pred[top], vfit)
         keep_n = config.get("no_of_retaining_parents", 0)
         order = np.argsort(-np.asarray(fit))[:max(keep_n, 1)]
         merged = [population[i] for i in order] + chosen
@@ -174,6 +184,7 @@ def run_surrogate_ga(rng, fitness, model, descriptors, config, pop_size=100, gen
         fit = mfit[best]
         _record(history, t0, gen, vfit, chosen, evals)
         history[-1]["surrogate_mae"] = err[-1]
+        history[-1]["screen_rank_corr"] = rank_corr
         if log:
             log(f"surrogate gen {gen}: best {history[-1]['best']:.2f} so far "
                 f"{history[-1]['best_so_far']:.2f} screen MAE {err[-1]:.2f} "

@@ -136,15 +136,30 @@ def cmd_compare(a):
     result = {"surrogate": sur, "config": {
         "pop_size": a.pop, "generations": a.generations, "reps": a.reps, "pool": a.pool,
         "verify": a.verify, "tumours": 3, "budget_gy": cfg["max_value"], "hours": HOURS}}
+    judge = D.SimulatorFitness(params, ga_t, a.reeval_reps, hours=HOURS, seed=a.seed + 99)
+
+    def reevaluate(hist):
+        """The best genome found, re-simulated with fresh replicates."""
+        best = max(hist, key=lambda h: h["best"])
+        return float(judge([np.asarray(best["best_genome"], np.float32)])[0])
+
+    def save():
+        with open(a.out, "w") as f:
+            json.dump(result, f, indent=1)
+
     if not a.skip_plain:
         fit = D.SimulatorFitness(params, ga_t, a.reps, hours=HOURS, seed=a.seed + 11)
         t0 = time.perf_counter()
         hist = D.run_plain_ga(np.random.default_rng(a.seed + 1), fit, cfg, pop_size=a.pop,
-                              generations=a.generations, n_slots=N_SLOTS, log=log)
+                              generations=a.generations, n_slots=N_SLOTS,
+                              time_limit=a.time_limit, log=log)
         result["plain"] = {"seconds": time.perf_counter() - t0, "history": hist,
                            "evaluations": fit.evaluations, "simulations": fit.simulations,
-                           "sim_seconds": fit.seconds}
+                           "sim_seconds": fit.seconds, "best_reevaluated": reevaluate(hist)}
         target = hist[-1]["best_so_far"]
+        save()
+        log(f"[plain] {result['plain']['seconds']:.1f} s, best {target:.2f}, re-evaluated "
+            f"{result['plain']['best_reevaluated']:.2f}")
     else:
         target = a.target
     if not a.skip_surrogate:
@@ -153,16 +168,21 @@ def cmd_compare(a):
         hist = D.run_surrogate_ga(np.random.default_rng(a.seed + 2), fit, model, ga_d, cfg,
                                   pop_size=a.pop, generations=a.generations, n_slots=N_SLOTS,
                                   pool=a.pool, verify=a.verify, refit_steps=a.refit_steps,
+                                  time_limit=a.time_limit,
                                   target=target if a.stop_at_target else None, log=log)
         result["surrogate_ga"] = {"seconds": time.perf_counter() - t0, "history": hist,
                                   "evaluations": fit.evaluations,
                                   "simulations": fit.simulations, "sim_seconds": fit.seconds}
+        result["surrogate_ga"]["best_reevaluated"] = reevaluate(hist)
         if target is not None:
             hit = [h for h in hist if h["best_so_far"] >= target]
             result["surrogate_ga"]["seconds_to_target"] = hit[0]["seconds"] if hit else None
             result["target"] = target
-    with open(a.out, "w") as f:
-        json.dump(result, f, indent=1)
+        log(f"[surrogate GA] {result['surrogate_ga']['seconds']:.1f} s, best "
+            f"{hist[-1]['best_so_far']:.2f}, re-evaluated "
+            f"{result['surrogate_ga']['best_reevaluated']:.2f}, to target "
+            f"{result['surrogate_ga'].get('seconds_to_target')}")
+    save()
     log(f"[compare] -> {a.out}")
 
 
@@ -193,6 +213,10 @@ def main(argv=None):
     c.add_argument("--skip-surrogate", action="store_true")
     c.add_argument("--target", type=float, default=None)
     c.add_argument("--stop-at-target", action="store_true")
+    c.add_argument("--time-limit", type=float, default=None,
+                   help="stop a GA after this many seconds (the generation running finishes)")
+    c.add_argument("--reeval-reps", type=int, default=100,
+                   help="replicates per tumour when re-simulating each run's best genome")
     c.add_argument("--seed", type=int, default=0)
     c.add_argument("--params", default=DEFAULT_PARAMS)
     a = p.parse_args(argv)
