@@ -28,6 +28,11 @@ sys.path.insert(0, os.path.join(ROOT, "emt6-ro_amd"))
 METRIC = "simulated tumour-hours/sec on 1 MI355X; GA evals/sec surrogate vs plain"
 FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 peak
 FLOP_PER_SITE_SUBSTEP = 36    # 3 substrates x (6 adds + 3 fma x 2)
+FP32_NONPACKED_TFLOPS = 78.6  # the same without packed FP32 (not emitted by this build)
+# HBM bytes per launch of the persistent kernel on this workload (one hour of
+# 20,000 simulations): FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy, from
+# profiles/persistent_kernel_batched_uniform_53x53.md.
+TRAFFIC_BYTES_PER_LAUNCH = 4.73e9 + 24.32e9
 
 
 def parse():
@@ -209,7 +214,9 @@ def main():
         "roofline": {
             "bound": "mfma", "pipe": "valu_fp32",
             "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+            "frac": achieved / FP32_PEAK_TFLOPS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,
+            "frac_of_nonpacked_fp32": achieved / FP32_NONPACKED_TFLOPS,
+            "hbm_gbps": TRAFFIC_BYTES_PER_LAUNCH / launch_s / 1e9,
             "kernel": "persistent_kernel", "launch_ms": launch_s * 1e3,
             "site_substeps_per_launch": (after - before) / max(args.steps, 1),
             "flop_per_site_substep": FLOP_PER_SITE_SUBSTEP,

@@ -44,6 +44,10 @@ class Mlp {
     EMT_HIPM(hipMemsetAsync(v_, 0, tot * 4, s_));
     EMT_HIPM(hipMemsetAsync(g_, 0, tot * 4, s_));
     for (int l = 0; l < kLayers; ++l) wb_[l] = dalloc<uint16_t>(sizes_[2 * l]);
+    // Split-K slices of the weight-gradient products: up to 16 slices of the
+    // largest weight matrix.
+    kwork_floats_ = 16 * std::max<int64_t>((int64_t)H_ * kin_, (int64_t)H_ * H_);
+    kwork_ = dalloc<float>(kwork_floats_);
     // PyTorch-style init: U(-1/sqrt(fan_in), 1/sqrt(fan_in)).
     std::mt19937_64 rng(seed);
     std::vector<float> h(tot, 0.f);
@@ -204,6 +208,7 @@ class Mlp {
     release(x_);
     for (int l = 0; l < kLayers; ++l) { release(z_[l]); release(h_[l]); release(dz_[l]); }
     release(y_); release(t_); release(dy_); release(sq_); release(ones_); release(xs_);
+    release(cwork_);
     bmax_ = B;
     x_ = dalloc<uint16_t>((size_t)B * kin_);
     xs_ = dalloc<float>((size_t)B * kin_);
@@ -214,6 +219,7 @@ class Mlp {
     }
     y_ = dalloc<float>(B); t_ = dalloc<float>(B); dy_ = dalloc<float>(B); sq_ = dalloc<float>(B);
     ones_ = dalloc<uint16_t>(B);
+    cwork_ = dalloc<float>((size_t)((B + 127) / 128) * H_);
     std::vector<uint16_t> one(B, 0x3f80);  // bf16 1.0
     EMT_HIPM(hipMemcpyAsync(ones_, one.data(), B * 2, hipMemcpyHostToDevice, s_));
     EMT_HIPM(hipStreamSynchronize(s_));
@@ -251,8 +257,8 @@ class Mlp {
 
   void backward(int B) {
     // Output layer: dw4 = dy^T h3, db4 = sum dy, dz3 = dy w4 gelu'(z3).
-    EMT_HIPM(mlp_colsum(h_[2], dy_, g_ + offs_[6], B, H_, s_));
-    EMT_HIPM(mlp_colsum(ones_, dy_, g_ + offs_[7], B, 1, s_));
+    EMT_HIPM(mlp_colsum(h_[2], dy_, g_ + offs_[6], B, H_, cwork_, s_));
+    EMT_HIPM(mlp_colsum(ones_, dy_, g_ + offs_[7], B, 1, cwork_, s_));
     EMT_HIPM(mlp_out_back(dy_, w_ + offs_[6], z_[2], dz_[2], B, H_, s_));
     for (int l = kLayers - 1; l >= 0; --l) {
       const uint16_t *in = l == 0 ? x_ : h_[l - 1];
@@ -263,8 +269,9 @@ class Mlp {
       q.B = in; q.sbk = K; q.sbn = 1;
       q.M = H_; q.N = K; q.K = B;
       q.epi = 2; q.f_out = g_ + offs_[2 * l]; q.ldc = K;
+      q.work = kwork_; q.work_floats = kwork_floats_;
       EMT_HIPM(mlp_gemm(q, s_));
-      EMT_HIPM(mlp_colsum(dz_[l], nullptr, g_ + offs_[2 * l + 1], B, H_, s_));
+      EMT_HIPM(mlp_colsum(dz_[l], nullptr, g_ + offs_[2 * l + 1], B, H_, cwork_, s_));
       if (l > 0) {
         // dz_{l-1} = (dz_l W_l) * gelu'(z_{l-1})
         MlpGemm r{};
@@ -295,6 +302,8 @@ class Mlp {
   int bmax_ = 0;
   uint16_t *x_ = nullptr, *z_[kLayers] = {}, *h_[kLayers] = {}, *dz_[kLayers] = {}, *ones_ = nullptr;
   float *y_ = nullptr, *t_ = nullptr, *dy_ = nullptr, *sq_ = nullptr, *xs_ = nullptr;
+  float *cwork_ = nullptr, *kwork_ = nullptr;
+  int64_t kwork_floats_ = 0;
   uint16_t *data_x_ = nullptr;
   float *data_t_ = nullptr;
   int n_data_ = 0;

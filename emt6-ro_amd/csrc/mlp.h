@@ -25,6 +25,9 @@ struct MlpGemm {
   const uint16_t *z_in;
   float *f_out;
   int64_t ldc;
+  // epi 2 only: scratch for split-K slices (may be null: no split).
+  float *work;
+  int64_t work_floats;
 };
 
 hipError_t mlp_gemm(const MlpGemm &q, hipStream_t s);
@@ -33,8 +36,9 @@ hipError_t mlp_gemv_out(const uint16_t *h, const float *w, const float *bias, fl
 hipError_t mlp_loss(const float *y, const float *t, float *dy, float *sq, int B, hipStream_t s);
 hipError_t mlp_out_back(const float *dy, const float *w, const uint16_t *z, uint16_t *dz, int B,
                         int K, hipStream_t s);
+// `work` holds ceil(B / 128) * K floats.
 hipError_t mlp_colsum(const uint16_t *x, const float *rowscale, float *out, int B, int K,
-                      hipStream_t s);
+                      float *work, hipStream_t s);
 hipError_t mlp_adam(float *w, float *m, float *v, const float *g, uint16_t *wb, int n, float lr,
                     float b1, float b2, float eps, int t, hipStream_t s);
 hipError_t mlp_to_bf16(const float *x, uint16_t *y, int64_t n, hipStream_t s);

@@ -59,6 +59,8 @@ struct GemmArgs {
   const uint16_t *z_in;        // kEpiBackGelu: pre-activation of this layer's input
   float *f_out;                // kEpiStoreF32
   int64_t ldc;
+  int k_chunk;                 // reduction slice of one blockIdx.z (multiple of kTK)
+  int64_t split_stride;        // kEpiStoreF32 with split K: floats between slices
 };
 
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
@@ -71,7 +73,10 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
 #pragma unroll
   for (int i = 0; i < 16; ++i) acc[i] = 0.f;
   const bool a_kfast = g.sak == 1, b_kfast = g.sbk == 1;
-  for (int k0 = 0; k0 < g.K; k0 += kTK) {
+  // Split K: slice blockIdx.z covers [kb, ke) of the reduction.
+  const int kb = blockIdx.z * g.k_chunk;
+  const int ke = min(g.K, kb + g.k_chunk);
+  for (int k0 = kb; k0 < ke; k0 += kTK) {
     // Stage a 64 x 32 tile of A and of B (rows m / n, k contiguous in LDS).
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
@@ -79,11 +84,11 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
       const int r = a_kfast ? idx / kTK : idx % kTile;
       const int c = a_kfast ? idx % kTK : idx / kTile;
       const int m = m0 + r, k = k0 + c;
-      As[r][c] = (m < g.M && k < g.K) ? g.A[m * g.sam + k * g.sak] : (uint16_t)0;
+      As[r][c] = (m < g.M && k < ke) ? g.A[m * g.sam + k * g.sak] : (uint16_t)0;
       const int rb = b_kfast ? idx / kTK : idx % kTile;
       const int cb = b_kfast ? idx % kTK : idx / kTile;
-      const int n = n0 + rb, kb = k0 + cb;
-      Bs[rb][cb] = (n < g.N && kb < g.K) ? g.B[kb * g.sbk + n * g.sbn] : (uint16_t)0;
+      const int n = n0 + rb, kk = k0 + cb;
+      Bs[rb][cb] = (n < g.N && kk < ke) ? g.B[kk * g.sbk + n * g.sbn] : (uint16_t)0;
     }
     __syncthreads();
 #pragma unroll
@@ -115,7 +120,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
     } else if (g.epi == kEpiBackGelu) {
       g.h_out[o] = to_bf16(v * gelu_grad(from_bf16(g.z_in[o])));
     } else {
-      g.f_out[o] = v;
+      g.f_out[(int64_t)blockIdx.z * g.split_stride + o] = v;
     }
   }
 }
@@ -149,21 +154,35 @@ __global__ void out_back_kernel(const float *dy, const float *w, const uint16_t 
   dz[i] = to_bf16(dy[b] * w[k] * gelu_grad(from_bf16(z[i])));
 }
 
-// Column sums: out[k] = sum_b x[b, k] (x bf16) or sum_b dy[b] h[b, k].
-__global__ __launch_bounds__(256) void colsum_kernel(const uint16_t *x, const float *rowscale,
-                                                     float *out, int B, int K) {
-  __shared__ float part[4][64];
+// out[i] = sum over s < nsplit of part[s * stride + i] (fixed order).
+__global__ void sum_slices_kernel(const float *part, int64_t stride, int nsplit, float *out,
+                                  int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = part[i];
+  for (int z = 1; z < nsplit; ++z) s += part[(int64_t)z * stride + i];
+  out[i] = s;
+}
+
+// Column sums, stage 1: block (bx, by) sums rows [by * kColRows, ...) of
+// columns [bx * 64, ...): out[k] = sum_b x[b, k] (x bf16), or sum_b dy[b] x[b, k].
+constexpr int kColRows = 128;
+__global__ __launch_bounds__(256) void colsum_part_kernel(const uint16_t *x, const float *rowscale,
+                                                          float *part, int B, int K) {
+  __shared__ float red[4][64];
   const int k = blockIdx.x * 64 + (threadIdx.x & 63), r0 = threadIdx.x >> 6;
+  const int b0 = blockIdx.y * kColRows, b1 = min(B, b0 + kColRows);
   float s = 0.f;
   if (k < K)
-    for (int b = r0; b < B; b += 4) {
+    for (int b = b0 + r0; b < b1; b += 4) {
       const float v = from_bf16(x[(int64_t)b * K + k]);
       s += rowscale ? rowscale[b] * v : v;
     }
-  part[r0][threadIdx.x & 63] = s;
+  red[r0][threadIdx.x & 63] = s;
   __syncthreads();
-  if (r0 == 0 && k < K) out[k] = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
-                                 part[3][threadIdx.x];
+  if (r0 == 0 && k < K)
+    part[(int64_t)blockIdx.y * K + k] =
+        ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
 }
 
 // Adam (Kingma & Ba, 2015) with bias correction; refreshes the bf16 copy.
@@ -237,8 +256,28 @@ hipError_t mlp_gemm(const MlpGemm &q, hipStream_t s) {
   g.M = q.M; g.N = q.N; g.K = q.K;
   g.epi = q.epi; g.bias = q.bias; g.act = q.act;
   g.z_out = q.z_out; g.h_out = q.h_out; g.z_in = q.z_in; g.f_out = q.f_out; g.ldc = q.ldc;
-  dim3 grid((q.N + mlp::kTile - 1) / mlp::kTile, (q.M + mlp::kTile - 1) / mlp::kTile);
+  const int tiles = ((q.N + mlp::kTile - 1) / mlp::kTile) * ((q.M + mlp::kTile - 1) / mlp::kTile);
+  // Split a long reduction over blockIdx.z when the output has few tiles
+  // (the weight gradients: 512 x 512 outputs, 4096-row batches); the
+  // slices land in `work` and are summed in a fixed order.
+  int split = 1;
+  if (q.epi == mlp::kEpiStoreF32 && q.work && q.work_floats > 0)
+    while (split < 16 && tiles * split * 2 <= 1024 && q.K / (split * 2) >= 4 * mlp::kTK &&
+           (int64_t)(split * 2) * q.M * q.ldc <= q.work_floats)
+      split *= 2;
+  g.k_chunk = ((q.K + split - 1) / split + mlp::kTK - 1) / mlp::kTK * mlp::kTK;
+  split = (q.K + g.k_chunk - 1) / g.k_chunk;
+  if (split > 1) {
+    g.f_out = q.work;
+    g.split_stride = (int64_t)q.M * q.ldc;
+  }
+  dim3 grid((q.N + mlp::kTile - 1) / mlp::kTile, (q.M + mlp::kTile - 1) / mlp::kTile, split);
   hipLaunchKernelGGL(mlp::gemm_bf16_kernel, grid, dim3(256), 0, s, g);
+  if (split > 1) {
+    const int64_t n = (int64_t)q.M * q.ldc;
+    hipLaunchKernelGGL(mlp::sum_slices_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, q.work,
+                       n, split, q.f_out, n);
+  }
   return hipGetLastError();
 }
 
@@ -261,8 +300,12 @@ hipError_t mlp_out_back(const float *dy, const float *w, const uint16_t *z, uint
 }
 
 hipError_t mlp_colsum(const uint16_t *x, const float *rowscale, float *out, int B, int K,
-                      hipStream_t s) {
-  hipLaunchKernelGGL(mlp::colsum_kernel, dim3(blocks(K, 64)), dim3(256), 0, s, x, rowscale, out, B, K);
+                      float *work, hipStream_t s) {
+  const int nb = (B + mlp::kColRows - 1) / mlp::kColRows;
+  hipLaunchKernelGGL(mlp::colsum_part_kernel, dim3(blocks(K, 64), nb), dim3(256), 0, s, x,
+                     rowscale, work, B, K);
+  hipLaunchKernelGGL(mlp::sum_slices_kernel, dim3(blocks(K, 256)), dim3(256), 0, s, work,
+                     (int64_t)K, nb, out, (int64_t)K);
   return hipGetLastError();
 }
 
