@@ -25,7 +25,7 @@ const VariantInfo *variant_table(int *count);
 inline bool variant_fits(const VariantInfo &v, int H, int W) {
   if (H + 2 > v.rs || W + 2 > v.rs || 4 * (H + W) > v.ps * v.nt) return false;
   if (v.kind == 0) return ((H + v.l - 1) / v.l) * W <= v.kmax * v.nt;
-  if (v.kind == 2) return true;  // items sized for any lattice up to rs - 2
+  if (v.kind >= 2) return true;  // items sized for any lattice up to rs - 2
   return H * W <= v.kmax * v.nt && (H - 2) * (W - 2) <= v.kf * v.nt;
 }
 

@@ -125,6 +125,118 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
   }
 }
 
+// Epilogue of one accumulator element (row m, column n, value v).
+__device__ __forceinline__ void gemm_store(const GemmArgs &g, int m, int n, float v) {
+  if (m >= g.M || n >= g.N) return;
+  const int64_t o = (int64_t)m * g.ldc + n;
+  if (g.epi == kEpiForward) {
+    const float z = v + g.bias[n];
+    const uint16_t zb = to_bf16(z);
+    if (g.z_out) g.z_out[o] = zb;
+    g.h_out[o] = g.act ? to_bf16(gelu(from_bf16(zb))) : zb;
+  } else if (g.epi == kEpiBackGelu) {
+    g.h_out[o] = to_bf16(v * gelu_grad(from_bf16(g.z_in[o])));
+  } else {
+    g.f_out[(int64_t)blockIdx.z * g.split_stride + o] = v;
+  }
+}
+
+// The same product on 128 x 128 tiles with 16-byte loads, for M, N, K and
+// the strides multiples of 8.  Each of the four waves owns a 64 x 64 quarter
+// as 2 x 2 blocks of v_mfma_f32_32x32x16_bf16 (lane l holds row l % 32 and
+// the 8 consecutive k from 8 (l / 32) of each operand).  AK / BK: operand A
+// (resp. B) is contiguous along k; otherwise along m (resp. n), and its
+// 8-element chunks are transposed into LDS.  The next k-tile is fetched into
+// registers while the current one is multiplied.
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int kT2 = 128, kK2 = 32, kLds2 = kK2 + 8;
+
+template <bool AK>
+__device__ __forceinline__ void load_chunks(const uint16_t *P, int64_t s_mn, int64_t s_k, int mn0,
+                                            int k0, int MN, int ke, uint4 v[2]) {
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int c = (int)threadIdx.x + r * 256;  // 512 chunks of 8 elements
+    int mn, k;
+    if (AK) { mn = mn0 + c / 4; k = k0 + (c % 4) * 8; }
+    else { k = k0 + c / 16; mn = mn0 + (c % 16) * 8; }
+    const bool ok = AK ? (mn < MN && k < ke) : (mn < MN && k < ke);
+    v[r] = ok ? *reinterpret_cast<const uint4 *>(P + mn * s_mn + k * s_k) : make_uint4(0, 0, 0, 0);
+  }
+}
+
+template <bool AK>
+__device__ __forceinline__ void store_chunks(uint16_t (*T)[kLds2], const uint4 v[2]) {
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int c = (int)threadIdx.x + r * 256;
+    if (AK) {
+      *reinterpret_cast<uint4 *>(&T[c / 4][(c % 4) * 8]) = v[r];
+    } else {
+      const int k = c / 16, mn = (c % 16) * 8;
+      const uint16_t *e = reinterpret_cast<const uint16_t *>(&v[r]);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) T[mn + i][k] = e[i];
+    }
+  }
+}
+
+template <bool AK, bool BK>
+__global__ __launch_bounds__(256) void gemm128_kernel(GemmArgs g) {
+  __shared__ __align__(16) uint16_t As[kT2][kLds2];
+  __shared__ __align__(16) uint16_t Bs[kT2][kLds2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = blockIdx.y * kT2, n0 = blockIdx.x * kT2;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const int kb = blockIdx.z * g.k_chunk;
+  const int ke = min(g.K, kb + g.k_chunk);
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.f;
+  // A(m, k) = A[m sam + k sak]; B(k, n) = B[k sbk + n sbn].
+  uint4 ra[2], rb[2];
+  load_chunks<AK>(g.A, g.sam, g.sak, m0, kb, g.M, ke, ra);
+  load_chunks<BK>(g.B, g.sbn, g.sbk, n0, kb, g.N, ke, rb);
+  for (int k0 = kb; k0 < ke; k0 += kK2) {
+    __syncthreads();
+    store_chunks<AK>(As, ra);
+    store_chunks<BK>(Bs, rb);
+    __syncthreads();
+    if (k0 + kK2 < ke) {
+      load_chunks<AK>(g.A, g.sam, g.sak, m0, k0 + kK2, g.M, ke, ra);
+      load_chunks<BK>(g.B, g.sbn, g.sbk, n0, k0 + kK2, g.N, ke, rb);
+    }
+#pragma unroll
+    for (int kk = 0; kk < kK2; kk += 16) {
+      const int kq = kk + (lane >> 5) * 8;
+      bf16x8 a[2], b[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        a[x] = *reinterpret_cast<const bf16x8 *>(&As[wm + x * 32 + (lane & 31)][kq]);
+        b[x] = *reinterpret_cast<const bf16x8 *>(&Bs[wn + x * 32 + (lane & 31)][kq]);
+      }
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[x], b[y], acc[x][y], 0, 0, 0);
+    }
+  }
+  // acc[x][y][j]: row 8 (j / 4) + 4 (lane / 32) + j % 4 of block x, column lane % 32 of block y.
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        gemm_store(g, m0 + wm + x * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3),
+                   n0 + wn + y * 32 + (lane & 31), acc[x][y][j]);
+}
+
 // y[b] = sum_k h[b, k] w[k] + bias   (one wave per row)
 __global__ __launch_bounds__(256) void gemv_out_kernel(const uint16_t *h, const float *w,
                                                        const float *bias, float *y, int B, int K) {
@@ -271,8 +383,22 @@ hipError_t mlp_gemm(const MlpGemm &q, hipStream_t s) {
     g.f_out = q.work;
     g.split_stride = (int64_t)q.M * q.ldc;
   }
-  dim3 grid((q.N + mlp::kTile - 1) / mlp::kTile, (q.M + mlp::kTile - 1) / mlp::kTile, split);
-  hipLaunchKernelGGL(mlp::gemm_bf16_kernel, grid, dim3(256), 0, s, g);
+  const bool a_k = q.sak == 1, b_k = q.sbk == 1;
+  const int64_t sa = a_k ? q.sam : q.sak, sb = b_k ? q.sbn : q.sbk;
+  const bool wide = q.M % 8 == 0 && q.N % 8 == 0 && q.K % 8 == 0 && sa % 8 == 0 &&
+                    sb % 8 == 0 && (a_k || q.sam == 1) && (b_k || q.sbn == 1) &&
+                    ((uintptr_t)q.A & 15) == 0 && ((uintptr_t)q.B & 15) == 0 &&
+                    g.k_chunk % mlp::kK2 == 0;
+  if (wide) {
+    dim3 grid2((q.N + mlp::kT2 - 1) / mlp::kT2, (q.M + mlp::kT2 - 1) / mlp::kT2, split);
+    if (a_k && b_k) hipLaunchKernelGGL((mlp::gemm128_kernel<true, true>), grid2, dim3(256), 0, s, g);
+    else if (a_k) hipLaunchKernelGGL((mlp::gemm128_kernel<true, false>), grid2, dim3(256), 0, s, g);
+    else if (b_k) hipLaunchKernelGGL((mlp::gemm128_kernel<false, true>), grid2, dim3(256), 0, s, g);
+    else hipLaunchKernelGGL((mlp::gemm128_kernel<false, false>), grid2, dim3(256), 0, s, g);
+  } else {
+    dim3 grid((q.N + mlp::kTile - 1) / mlp::kTile, (q.M + mlp::kTile - 1) / mlp::kTile, split);
+    hipLaunchKernelGGL(mlp::gemm_bf16_kernel, grid, dim3(256), 0, s, g);
+  }
   if (split > 1) {
     const int64_t n = (int64_t)q.M * q.ldc;
     hipLaunchKernelGGL(mlp::sum_slices_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, q.work,

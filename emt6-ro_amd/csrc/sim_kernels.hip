@@ -96,6 +96,7 @@ __global__ __launch_bounds__(NT, WPE) void persistent_kernel(RunArgs a) {
     int roi[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) roi[q] = sm.roi[q];
+    diff.set_sim(qi);
     diff.plan(roi, sm);
     for (uint32_t k = 0; k < nsteps; ++k) {
       const uint32_t t = step0 + k;
@@ -142,6 +143,10 @@ struct DiffSel<1, NT, A, B, RS, PS> {
 template <int NT, int A, int B, int RS, int PS>
 struct DiffSel<2, NT, A, B, RS, PS> {
   using T = MultiBandDiffuser<NT, RS, PS>;
+};
+template <int NT, int A, int B, int RS, int PS>
+struct DiffSel<3, NT, A, B, RS, PS> {
+  using T = MultiBandDiffuser<NT, RS, PS, true>;
 };
 
 // ---------------------------------------------------------------------------
@@ -303,7 +308,9 @@ hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds
 
 }  // namespace
 
-// Kernel variants: id, kind (0 banded in place, 1 ping-pong site-parallel, 2 banded with the band height chosen per ROI),
+// Kernel variants: id, kind (0 banded in place, 1 ping-pong site-parallel,
+// 2 banded with the band height chosen per ROI, 3 as 2 with the first items
+// rotated across waves per simulation),
 // workgroup size, A, B (kind 0: band rows, items per thread; kind 1: Jacobi
 // and final sites per thread), ring-save slots per thread, LDS row stride
 // (lattices up to stride-2 on a side), minimum waves per SIMD the compiler
@@ -328,7 +335,11 @@ hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds
   X(16, 2, 256, 0, 0, 2, 56, 1)   \
   X(17, 2, 256, 0, 0, 2, 36, 4)   \
   X(18, 2, 128, 0, 0, 3, 36, 1)   \
-  X(19, 2, 256, 0, 0, 3, 88, 2)
+  X(19, 2, 256, 0, 0, 3, 88, 2)   \
+  X(20, 3, 256, 0, 0, 2, 56, 4)   \
+  X(21, 3, 128, 0, 0, 4, 56, 1)   \
+  X(22, 3, 256, 0, 0, 2, 36, 4)   \
+  X(23, 3, 256, 0, 0, 3, 88, 2)
 
 const VariantInfo *variant_table(int *count) {
   static const VariantInfo table[] = {

@@ -516,13 +516,25 @@ struct MultiBand {
   static constexpr int kSlotsMax = kSlots(3);
 };
 
-// Band heights tried per ROI: 3, 4 and 6 rows.
-template <int NT, int RS, int PS>
+// Band heights tried per ROI: 3, 4 and 6 rows.  With ROT the waves that get
+// the first items rotate with the simulation's queue position, so that
+// workgroups sharing a CU do not all load the SIMD of their first wave.
+template <int NT, int RS, int PS, bool ROT = false>
 struct MultiBandDiffuser {
   static constexpr int kPlanes = 3;
   static constexpr int KM = MultiBand<NT, RS>::kSlotsMax;
+  static constexpr int NW = NT / 64;
   uint32_t jac[KM], fin[KM];
   int L = 4;
+  int rot = 0;
+
+  __device__ void set_sim(int queue_pos) { rot = ROT ? queue_pos % NW : 0; }
+
+  // Item index of this thread in slot k.
+  __device__ int thread_q(int k) const {
+    const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
+    return ((w + NW - rot) % NW) * 64 + lane + k * NT;
+  }
 
   // Items of band height Lc for the chords in LDS; with item_q >= 0 also
   // returns that item's band and first column.
@@ -579,7 +591,7 @@ struct MultiBandDiffuser {
     L = best;
 #pragma unroll
     for (int k = 0; k < KM; ++k) {
-      const int q = (int)threadIdx.x + k * NT;
+      const int q = thread_q(k);
       int b = -1, j = 0;
       count_items(chord, hm, wm, L, q, b, j);
       if (b < 0) continue;
@@ -594,7 +606,7 @@ struct MultiBandDiffuser {
     const int nf = ((h + L - 1) / L) * w;
 #pragma unroll
     for (int k = 0; k < KM; ++k) {
-      const int q = (int)threadIdx.x + k * NT;
+      const int q = thread_q(k);
       if (q >= nf) continue;
       const int band = q / w, j = 1 + (q - band * w), i0 = 1 + band * L;
       uint32_t m = 0u;
@@ -626,6 +638,7 @@ template <int NT, int L, int KMAX, int RS, int PS>
 struct BandedDiffuser {
   static constexpr int kPlanes = 3;  // substrate planes in LDS
   uint32_t jac[KMAX], fin[KMAX];
+  __device__ void set_sim(int) {}
   __device__ void plan(const int roi[5], Shared &sm) { plan_diffusion<NT, L, KMAX>(roi, jac, fin, sm); }
   __device__ void run(const DevParams &p, float *sub, int PV, const int roi[5], uint32_t &upd) {
     diffuse<NT, L, KMAX, RS, PS>(p, sub, PV, roi, jac, fin, upd);
@@ -722,6 +735,7 @@ template <int NT, int KJ, int KF, int RS, int PS>
 struct PingPongDiffuser {
   static constexpr int kPlanes = 6;  // two buffers of three planes
   uint32_t jac[KJ], fin[KF];
+  __device__ void set_sim(int) {}
   __device__ void plan(const int roi[5], Shared &sm) { plan_sites<NT, KJ, KF>(roi, RS, jac, fin, sm); }
   __device__ void run(const DevParams &p, float *A, int PV, const int roi[5], uint32_t &upd) {
     const int r0 = roi[0], c0 = roi[1], h = roi[2], w = roi[3], dist = roi[4];

@@ -89,7 +89,9 @@ class Batch:
 
 def variants():
     """Compiled kernel shapes: dicts of id, kind (0 banded in place, 1
-    ping-pong site-parallel diffusion), nt (workgroup size), l (band rows),
+    ping-pong site-parallel, 2 banded with the band height chosen per ROI, 3
+    as 2 with the first items rotated across waves), nt (workgroup size),
+    l (band rows),
     kmax (items or Jacobi sites per thread), kf (final sites per thread),
     ps (ring slots per thread), rs (LDS row stride)."""
     n = lib.emt6ro_variant_info(-1, None)
@@ -107,6 +109,6 @@ def fits(v, H, W):
         return False
     if v["kind"] == 0:
         return -(-H // v["l"]) * W <= v["kmax"] * v["nt"]
-    if v["kind"] == 2:
+    if v["kind"] >= 2:
         return True
     return H * W <= v["kmax"] * v["nt"] and (H - 2) * (W - 2) <= v["kf"] * v["nt"]
