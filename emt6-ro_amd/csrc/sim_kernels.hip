@@ -146,8 +146,9 @@ struct DiffSel<2, NT, A, B, RS, PS> {
 };
 template <int NT, int A, int B, int RS, int PS>
 struct DiffSel<3, NT, A, B, RS, PS> {
-  using T = MultiBandDiffuser<NT, RS, PS, true>;
+  using T = LaneRunDiffuser<NT, RS, PS, A>;
 };
+
 
 // ---------------------------------------------------------------------------
 // Per-step path.
@@ -309,12 +310,13 @@ hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds
 }  // namespace
 
 // Kernel variants: id, kind (0 banded in place, 1 ping-pong site-parallel,
-// 2 banded with the band height chosen per ROI, 3 as 2 with the first items
-// rotated across waves per simulation),
+// 2 banded with the band height chosen per ROI, 3 as 2 in the lane-run
+// layout with horizontal neighbours from adjacent lanes),
 // workgroup size, A, B (kind 0: band rows, items per thread; kind 1: Jacobi
-// and final sites per thread), ring-save slots per thread, LDS row stride
-// (lattices up to stride-2 on a side), minimum waves per SIMD the compiler
-// must allow (bounds the registers so that many workgroups fit a CU).
+// and final sites per thread; kind 3: A = lane slots per thread), ring-save
+// slots per thread, LDS row stride (lattices up to stride-2 on a side),
+// minimum waves per SIMD the compiler must allow (bounds the registers so
+// that many workgroups fit a CU).
 #define EMT6RO_VARIANTS(X)        \
   X(0, 0, 128, 4, 6, 4, 56, 1)    \
   X(1, 0, 256, 4, 3, 2, 56, 1)    \
@@ -336,16 +338,18 @@ hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds
   X(17, 2, 256, 0, 0, 2, 36, 4)   \
   X(18, 2, 128, 0, 0, 3, 36, 1)   \
   X(19, 2, 256, 0, 0, 3, 88, 2)   \
-  X(20, 3, 256, 0, 0, 2, 56, 4)   \
-  X(21, 3, 128, 0, 0, 4, 56, 1)   \
-  X(22, 3, 256, 0, 0, 2, 36, 4)   \
-  X(23, 3, 256, 0, 0, 3, 88, 2)
+  X(20, 3, 512, 2, 0, 1, 56, 4)   \
+  X(21, 3, 512, 2, 0, 1, 56, 2)   \
+  X(22, 3, 256, 3, 0, 2, 56, 2)   \
+  X(23, 3, 256, 1, 0, 2, 36, 4)   \
+  X(24, 3, 512, 3, 0, 1, 88, 2)
 
 const VariantInfo *variant_table(int *count) {
   static const VariantInfo table[] = {
 #define X(id, kind, nt, A, B, ps, rs, wpe) \
   {id, kind, nt, kind == 0 ? A : kind == 1 ? 1 : 8,                                  \
-   kind == 0 ? B : kind == 1 ? A : dev::MultiBand<nt, rs>::kSlotsMax, kind == 1 ? B : 0, ps, rs},
+   kind == 0 ? B : kind == 1 ? A : kind == 2 ? dev::MultiBand<nt, rs>::kSlotsMax : A,      \
+   kind == 1 ? B : 0, ps, rs},
       EMT6RO_VARIANTS(X)
 #undef X
   };

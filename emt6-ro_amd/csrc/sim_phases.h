@@ -295,8 +295,9 @@ __device__ void find_roi(const uint8_t *st, int H, int W, Shared &sm) {
 // read up to L-1 rows below the temporary grid; those rows only feed results
 // that are not stored, and the LDS layout keeps them inside the allocation.
 //
-// Stencil, per substrate, in this order (no other fused operations):
-//   V(c) = up(c) + down(c),  o = V(c) + (left + right),  d = V(c-1) + V(c+1)
+// Stencil, per substrate, in this order (no other fused operations), with
+// H(row) = left + right, the row's neighbours of the centre column:
+//   o = (up + down) + H(mid),  d = H(up row) + H(down row)
 //   L = fma(d, 1/sqrt2, o);  L = fma(-F, x, L);  x' = fma(k, L, x)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool masked(int i, int j, int hm, int wm, int d4) {
@@ -380,9 +381,31 @@ __device__ void plan_diffusion(const int roi[5], uint32_t jac[KMAX], uint32_t fi
   __syncthreads();
 }
 
+// Horizontal neighbours through the wave: DPP wave_shr:1 gives lane i the
+// value of lane i - 1, wave_shl:1 that of lane i + 1.
+__device__ __forceinline__ float from_left_lane(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_right_lane(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+// Item word of the lane-run layout (LaneRunDiffuser): column + 1 (bits 0-7),
+// first band row (8-15), mask of rows to write (16-29), valid (31).
+constexpr uint32_t kItemValid = 1u << 31;
+__device__ __forceinline__ uint32_t pack_lane_item(int j, int i0, uint32_t mask) {
+  return (uint32_t)(j + 1) | ((uint32_t)i0 << 8) | (mask << 16) | kItemValid;
+}
+
 // One diffusion step on the padded lattice in LDS (planes PV floats apart,
-// row stride RS).  `jac` and `fin` come from plan_diffusion for the same ROI.
-template <int NT, int L, int KMAX, int RS, int PS>
+// row stride RS).  `jac` and `fin` come from the plan for the same ROI.
+// LANES = false: items are columns of bands and read their horizontal
+// neighbours from LDS (plan_diffusion / MultiBand layout).  LANES = true:
+// items are laid out in runs of consecutive lanes holding consecutive
+// columns of one band, with a read-only item at each end of a run (mask 0),
+// so every written site finds its left and right neighbours in the adjacent
+// lanes and reads only its own column from LDS.
+template <int NT, int L, int KMAX, int RS, int PS, bool LANES = false>
 __device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5],
                         const uint32_t jac[KMAX], const uint32_t fin[KMAX], uint32_t &updates) {
   const int r0 = roi[0], c0 = roi[1], h = roi[2], w = roi[3], dist = roi[4];
@@ -436,7 +459,10 @@ __device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5]
   {
     uint32_t na = 0, nf = 0;
 #pragma unroll
-    for (int k = 0; k < KMAX; ++k) { na += __popc(jac[k] >> 16); nf += __popc(fin[k] >> 16); }
+    for (int k = 0; k < KMAX; ++k) {
+      na += __popc((jac[k] >> 16) & 0x3fffu);
+      nf += __popc((fin[k] >> 16) & 0x3fffu);
+    }
     updates += na * (uint32_t)(total - 1) + nf;
   }
   const float R2 = (float)0.70710678118654752440;
@@ -450,8 +476,9 @@ __device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5]
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
       const uint32_t it = final_sub ? fin[k] : jac[k];
-      if ((it >> 16) == 0u) continue;
-      const int j = (int)(it & 0xffu), i0 = (int)((it >> 8) & 0xffu);
+      if (LANES ? (it & kItemValid) == 0u : (it >> 16) == 0u) continue;
+      const int j = LANES ? (int)(it & 0xffu) - 1 : (int)(it & 0xffu);
+      const int i0 = (int)((it >> 8) & 0xffu);
       const int a0 = origin + (i0 - 1) * RS + j;
 #pragma unroll
       for (int s = 0; s < 3; ++s) {
@@ -459,12 +486,21 @@ __device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5]
         const float kk = p.k[s];
         // Rolling window down the column: centre values of the rows above,
         // at and below, and their horizontal neighbour sums hu, hm, hd.
-        float uc = P[0], hu = P[-1] + P[1];
-        float mc = P[RS], hm = P[RS - 1] + P[RS + 1];
+        float uc, hu, mc, hm;
+        if (LANES) {
+          uc = P[0];
+          mc = P[RS];
+          hu = from_left_lane(uc) + from_right_lane(uc);
+          hm = from_left_lane(mc) + from_right_lane(mc);
+        } else {
+          uc = P[0]; hu = P[-1] + P[1];
+          mc = P[RS]; hm = P[RS - 1] + P[RS + 1];
+        }
 #pragma unroll
         for (int l = 0; l < L; ++l) {
           const float *dn = P + (l + 2) * RS;
-          const float dc = dn[0], hd = dn[-1] + dn[1];
+          const float dc = dn[0];
+          const float hd = LANES ? from_left_lane(dc) + from_right_lane(dc) : dn[-1] + dn[1];
           const float o = (uc + dc) + hm;
           const float dd = hu + hd;
           float Lp = fmaf(dd, R2, o);
@@ -479,9 +515,10 @@ __device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5]
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
       const uint32_t it = final_sub ? fin[k] : jac[k];
-      const uint32_t m = it >> 16;
+      const uint32_t m = (it >> 16) & 0x3fffu;
       if (m == 0u) continue;
-      const int j = (int)(it & 0xffu), i0 = (int)((it >> 8) & 0xffu);
+      const int j = LANES ? (int)(it & 0xffu) - 1 : (int)(it & 0xffu);
+      const int i0 = (int)((it >> 8) & 0xffu);
       float *P = sub + origin + i0 * RS + j;
 #pragma unroll
       for (int l = 0; l < L; ++l) {
@@ -516,25 +553,20 @@ struct MultiBand {
   static constexpr int kSlotsMax = kSlots(3);
 };
 
-// Band heights tried per ROI: 3, 4 and 6 rows.  With ROT the waves that get
-// the first items rotate with the simulation's queue position, so that
-// workgroups sharing a CU do not all load the SIMD of their first wave.
-template <int NT, int RS, int PS, bool ROT = false>
+// Band heights tried per ROI: 3, 4 and 6 rows.  (The hardware places the
+// waves of co-resident workgroups on different SIMDs, so packing the items
+// into the first waves of each workgroup does not load one SIMD.)
+template <int NT, int RS, int PS>
 struct MultiBandDiffuser {
   static constexpr int kPlanes = 3;
   static constexpr int KM = MultiBand<NT, RS>::kSlotsMax;
-  static constexpr int NW = NT / 64;
   uint32_t jac[KM], fin[KM];
   int L = 4;
-  int rot = 0;
 
-  __device__ void set_sim(int queue_pos) { rot = ROT ? queue_pos % NW : 0; }
+  __device__ void set_sim(int) {}
 
   // Item index of this thread in slot k.
-  __device__ int thread_q(int k) const {
-    const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
-    return ((w + NW - rot) % NW) * 64 + lane + k * NT;
-  }
+  __device__ int thread_q(int k) const { return (int)threadIdx.x + k * NT; }
 
   // Items of band height Lc for the chords in LDS; with item_q >= 0 also
   // returns that item's band and first column.
@@ -624,6 +656,166 @@ struct MultiBandDiffuser {
 #pragma unroll
     for (int k = 0; k < K; ++k) { j2[k] = jac[k]; f2[k] = fin[k]; }
     diffuse<NT, LL, K, RS, PS>(p, sub, PV, roi, j2, f2, upd);
+  }
+
+  __device__ void run(const DevParams &p, float *sub, int PV, const int roi[5], uint32_t &upd) {
+    if (L == 3) run_l<3>(p, sub, PV, roi, upd);
+    else if (L == 4) run_l<4>(p, sub, PV, roi, upd);
+    else run_l<6>(p, sub, PV, roi, upd);
+  }
+};
+
+// Multi-band diffusion in the lane-run layout (diffuse<..., LANES = true>).
+// The band height (3, 4 or 6 rows) is chosen per ROI like MultiBandDiffuser.
+// A band's columns that hold unmasked sites form a run; the run, with one
+// read-only column on each side, occupies consecutive lanes of one wave.  A
+// run that does not fit the rest of a wave continues in the next one, the
+// two columns at the cut being repeated so that both parts keep a read-only
+// column at each end; a wave with fewer than 3 lanes left is not used.
+template <int NT, int RS, int KCAP = 8>
+struct LaneRuns {
+  // Lanes a height-L layout can need: every band one run of at most RS
+  // columns plus 2 read-only ones, 2 repeated columns per wave cut, up to 2
+  // lanes lost per wave.
+  static constexpr int kMaxLanes(int L) {
+    return ((RS - 2 + L - 1) / L) * (RS + 2) + 4 * (((RS - 2 + L - 1) / L) * (RS + 2) / 60 + 2);
+  }
+  // Lane slots per thread for height L, at most KCAP (the results of every
+  // slot stay in registers across a barrier); a plan uses a height only if
+  // its layout fits.  Height 6 always fits.
+  static constexpr int kSlots(int L) {
+    return (kMaxLanes(L) + NT - 1) / NT < KCAP ? (kMaxLanes(L) + NT - 1) / NT : KCAP;
+  }
+  static constexpr int kSlotsMax = kSlots(3) > kSlots(6) ? kSlots(3) : kSlots(6);
+  static_assert(KCAP * NT >= kMaxLanes(6), "height-6 layouts must fit");
+
+  struct Lane {
+    int run, col;
+    bool owner;
+  };
+
+  // Places runs r = 0..nr-1 (columns [lo, hi) from runfn); returns the
+  // lanes used and, for lane q >= 0, what it holds (run -1: nothing).
+  template <typename RunFn>
+  __device__ static int layout(int nr, const RunFn &runfn, int q, Lane &out) {
+    out.run = -1;
+    int pos = 0;
+    for (int r = 0; r < nr; ++r) {
+      int lo, hi;
+      runfn(r, lo, hi);
+      if (hi <= lo) continue;
+      int c = lo - 1;
+      for (;;) {
+        const int room = 64 - (pos & 63);
+        if (room < 3) {
+          pos += room;
+          continue;
+        }
+        const int take = min(room, hi + 1 - c);
+        if (q >= pos && q < pos + take) {
+          out.run = r;
+          out.col = c + (q - pos);
+          out.owner = q > pos && q < pos + take - 1;
+        }
+        pos += take;
+        if (c + take - 1 >= hi) break;
+        c += take - 2;
+      }
+    }
+    return pos;
+  }
+};
+
+template <int NT, int RS, int PS, int KCAP>
+struct LaneRunDiffuser {
+  using LR = LaneRuns<NT, RS, KCAP>;
+  static constexpr int kPlanes = 3;
+  static constexpr int KM = LR::kSlotsMax;
+  uint32_t jac[KM], fin[KM];
+  int L = 4;
+
+  __device__ void set_sim(int) {}
+
+  __device__ void plan(const int roi[5], Shared &sm) {
+    const int h = roi[2], w = roi[3], dist = roi[4];
+#pragma unroll
+    for (int k = 0; k < KM; ++k) { jac[k] = 0u; fin[k] = 0u; }
+    if (h == 0 || w == 0) return;
+    const int hm = h + 2, wm = w + 2, d4 = 4 * dist * dist;
+    int32_t *chord = sm.red;  // per mask-box row: first | (past-last << 16)
+    for (int i = threadIdx.x; i < hm; i += NT) {
+      const int ai = 2 * (i + 1) - hm;
+      const int rem = d4 - ai * ai;
+      int a = wm, e = 0;
+      if (rem > 0)
+        for (int j = 0; j < wm; ++j) {
+          const int bj = 2 * (j + 1) - wm;
+          if (bj * bj < rem) { a = min(a, j); e = max(e, j + 1); }
+        }
+      chord[i] = e > a ? (a | (e << 16)) : 0;
+    }
+    __syncthreads();
+    int Lc = 6;
+    auto jac_runs = [&](int b, int &lo, int &hi) {
+      int a = wm, e = 0;
+      for (int l = 0; l < Lc && b * Lc + l < hm; ++l) {
+        const int c = chord[b * Lc + l];
+        if (c) { a = min(a, c & 0xffff); e = max(e, c >> 16); }
+      }
+      lo = a;
+      hi = e > a ? e : a;
+    };
+    auto fin_runs = [&](int, int &lo, int &hi) { lo = 1; hi = w + 1; };
+    typename LR::Lane dummy;
+    int best = 6, best_cost = -1;
+    {
+      const int lcs[3] = {3, 4, 6};
+      const int slots[3] = {LR::kSlots(3), LR::kSlots(4), LR::kSlots(6)};
+#pragma unroll
+      for (int ci = 0; ci < 3; ++ci) {
+        Lc = lcs[ci];
+        const int n = LR::layout((hm + Lc - 1) / Lc, jac_runs, -1, dummy);
+        const int nf = LR::layout((h + Lc - 1) / Lc, fin_runs, -1, dummy);
+        if (n > slots[ci] * NT || nf > slots[ci] * NT) continue;
+        const int cost = ((n + 63) / 64) * (24 * Lc + 30);
+        if (best_cost < 0 || cost < best_cost) { best = Lc; best_cost = cost; }
+      }
+    }
+    L = Lc = best;
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      typename LR::Lane ln;
+      LR::layout((hm + L - 1) / L, jac_runs, (int)threadIdx.x + k * NT, ln);
+      if (ln.run >= 0) {
+        const int i0 = ln.run * L;
+        uint32_t m = 0u;
+        if (ln.owner)
+          for (int l = 0; l < L; ++l) {
+            const int i = i0 + l;
+            if (i < hm && !masked(i, ln.col, hm, wm, d4)) m |= 1u << l;
+          }
+        jac[k] = pack_lane_item(ln.col, i0, m);
+      }
+      LR::layout((h + L - 1) / L, fin_runs, (int)threadIdx.x + k * NT, ln);
+      if (ln.run >= 0) {
+        const int i0 = 1 + ln.run * L;
+        uint32_t m = 0u;
+        if (ln.owner)
+          for (int l = 0; l < L; ++l)
+            if (i0 + l <= h) m |= 1u << l;
+        fin[k] = pack_lane_item(ln.col, i0, m);
+      }
+    }
+    __syncthreads();
+  }
+
+  template <int LL>
+  __device__ void run_l(const DevParams &p, float *sub, int PV, const int roi[5], uint32_t &upd) {
+    constexpr int K = LR::kSlots(LL);
+    uint32_t j2[K], f2[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) { j2[k] = jac[k]; f2[k] = fin[k]; }
+    diffuse<NT, LL, K, RS, PS, true>(p, sub, PV, roi, j2, f2, upd);
   }
 
   __device__ void run(const DevParams &p, float *sub, int PV, const int roi[5], uint32_t &upd) {

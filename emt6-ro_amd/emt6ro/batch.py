@@ -90,8 +90,8 @@ class Batch:
 def variants():
     """Compiled kernel shapes: dicts of id, kind (0 banded in place, 1
     ping-pong site-parallel, 2 banded with the band height chosen per ROI, 3
-    as 2 with the first items rotated across waves), nt (workgroup size),
-    l (band rows),
+    as 2 with horizontal neighbours from adjacent lanes), nt (workgroup
+    size), l (band rows),
     kmax (items or Jacobi sites per thread), kf (final sites per thread),
     ps (ring slots per thread), rs (LDS row stride)."""
     n = lib.emt6ro_variant_info(-1, None)

@@ -77,11 +77,14 @@ def test_forward_matches_oracle(S, n):
     assert e64 < 5e-2
 
 
-def test_gradients_match_oracle(S):
-    log("[surrogate] gradients")
+@pytest.mark.parametrize("n", [700, 768])
+def test_gradients_match_oracle(S, n):
+    """n = 700 runs the 64 x 64 GEMM tiles, n = 768 (all dimensions multiples
+    of 8) the 128 x 128 tiles with transposed operand loads and split K."""
+    log(f"[surrogate] gradients n={n}")
     net = S.MLP(N_IN, HID, seed=3)
     p = net.get_params().astype(np.float64)
-    X, t = _data(700, 7)
+    X, t = _data(n, 7)
     loss, g = net.grads(X, t)
     l16, g16 = M.backward(p, X.astype(np.float64), t.astype(np.float64), N_IN, HID,
                           bf16_mode=True)
