@@ -9,6 +9,7 @@
 //                       launch per phase per step, state in HBM between them.
 //   count_living_kernel : living cells per simulation (statistics.cu:9-38).
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "kernels.h"
 #include "sim_phases.h"
 
@@ -145,8 +146,22 @@ struct DiffSel<2, NT, A, B, RS, PS> {
   using T = MultiBandDiffuser<NT, RS, PS>;
 };
 template <int NT, int A, int B, int RS, int PS>
+struct DiffSel<4, NT, A, B, RS, PS> {
+  using T = typename std::conditional<B == 0, PairRunDiffuser<NT, RS, PS, A>,
+                                      PairRunDiffuser<NT, RS, PS, A, 2, 4, 8, 1>>::type;
+};
+// Lane-run band heights and policy by code B: 0 3/4/6, 1 4/6/8 (fewest
+// instructions); 2 4/6/8, 3 2/3/4, 4 3/4/6 (lowest bands that fit).
+template <int B> struct LaneRunShape;
+template <> struct LaneRunShape<0> { static constexpr int h1 = 3, h2 = 4, h3 = 6, pol = 0; };
+template <> struct LaneRunShape<1> { static constexpr int h1 = 4, h2 = 6, h3 = 8, pol = 0; };
+template <> struct LaneRunShape<2> { static constexpr int h1 = 4, h2 = 6, h3 = 8, pol = 1; };
+template <> struct LaneRunShape<3> { static constexpr int h1 = 2, h2 = 3, h3 = 4, pol = 1; };
+template <> struct LaneRunShape<4> { static constexpr int h1 = 3, h2 = 4, h3 = 6, pol = 1; };
+template <int NT, int A, int B, int RS, int PS>
 struct DiffSel<3, NT, A, B, RS, PS> {
-  using T = LaneRunDiffuser<NT, RS, PS, A>;
+  using S = LaneRunShape<B>;
+  using T = LaneRunDiffuser<NT, RS, PS, A, S::h1, S::h2, S::h3, S::pol>;
 };
 
 
@@ -311,9 +326,13 @@ hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds
 
 // Kernel variants: id, kind (0 banded in place, 1 ping-pong site-parallel,
 // 2 banded with the band height chosen per ROI, 3 as 2 in the lane-run
-// layout with horizontal neighbours from adjacent lanes),
+// layout with horizontal neighbours from adjacent lanes, 4 as 3 with two
+// columns per lane),
 // workgroup size, A, B (kind 0: band rows, items per thread; kind 1: Jacobi
-// and final sites per thread; kind 3: A = lane slots per thread), ring-save
+// and final sites per thread; kind 3: A = lane slots per thread, B = band
+// heights and policy (LaneRunShape); kind 4:
+// A = lane slots per thread, B = heights 4/6/8 (0) or 2/4/8 preferring low
+// bands (1)), ring-save
 // slots per thread, LDS row stride (lattices up to stride-2 on a side),
 // minimum waves per SIMD the compiler must allow (bounds the registers so
 // that many workgroups fit a CU).
@@ -342,7 +361,21 @@ hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds
   X(21, 3, 512, 2, 0, 1, 56, 2)   \
   X(22, 3, 256, 3, 0, 2, 56, 2)   \
   X(23, 3, 256, 1, 0, 2, 36, 4)   \
-  X(24, 3, 512, 3, 0, 1, 88, 2)
+  X(24, 3, 512, 3, 0, 1, 88, 2)   \
+  X(25, 3, 256, 2, 1, 2, 56, 4)   \
+  X(26, 3, 256, 2, 1, 2, 56, 3)   \
+  X(27, 4, 256, 1, 0, 2, 56, 4)   \
+  X(28, 4, 256, 1, 0, 2, 56, 3)   \
+  X(29, 4, 128, 2, 0, 4, 56, 4)   \
+  X(30, 4, 256, 1, 0, 2, 36, 4)   \
+  X(31, 4, 256, 3, 0, 3, 88, 2)   \
+  X(32, 3, 256, 2, 2, 2, 56, 4)   \
+  X(33, 4, 256, 2, 1, 2, 56, 4)   \
+  X(34, 3, 256, 2, 2, 2, 36, 4)   \
+  X(35, 3, 256, 4, 3, 2, 56, 4)   \
+  X(36, 3, 256, 3, 4, 2, 56, 4)   \
+  X(37, 3, 512, 1, 2, 1, 56, 4)   \
+  X(38, 3, 256, 2, 2, 2, 56, 3)
 
 const VariantInfo *variant_table(int *count) {
   static const VariantInfo table[] = {

@@ -405,55 +405,79 @@ __device__ __forceinline__ uint32_t pack_lane_item(int j, int i0, uint32_t mask)
 // columns of one band, with a read-only item at each end of a run (mask 0),
 // so every written site finds its left and right neighbours in the adjacent
 // lanes and reads only its own column from LDS.
-template <int NT, int L, int KMAX, int RS, int PS, bool LANES = false>
-__device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5],
-                        const uint32_t jac[KMAX], const uint32_t fin[KMAX], uint32_t &updates) {
-  const int r0 = roi[0], c0 = roi[1], h = roi[2], w = roi[3], dist = roi[4];
-  if (h == 0 || w == 0) return;
-  const int tid = threadIdx.x;
-  const int hm = h + 2, wm = w + 2;
-  const int d4 = 4 * dist * dist;
-  // Padded coordinates of the temporary grid: rows R0..R1, cols C0..C1.
-  const int R0 = r0 - 1, R1 = r0 + h + 2, C0 = c0 - 1, C1 = c0 + w + 2;
-  const int nper = 4 * w + 4 * h + 16;
+// The reference's temporary grid (diffusion.cu:150-162) made in place: the
+// two rings around the ROI are saved to registers and set to their
+// temporary values (outer ring 0, masked sites the external levels), masked
+// ROI sites get the external levels, and restore() puts the rings back.
+template <int NT, int RS, int PS>
+struct TempGrid {
   float sv[PS][3];
   int sidx[PS];
+
+  __device__ void setup(const DevParams &p, float *sub, int PV, const int roi[5]) {
+    const int r0 = roi[0], c0 = roi[1], h = roi[2], w = roi[3], dist = roi[4];
+    const int tid = threadIdx.x;
+    const int hm = h + 2, wm = w + 2;
+    const int d4 = 4 * dist * dist;
+    // Padded coordinates of the temporary grid: rows R0..R1, cols C0..C1.
+    const int R0 = r0 - 1, R1 = r0 + h + 2, C0 = c0 - 1, C1 = c0 + w + 2;
+    const int nper = 4 * w + 4 * h + 16;
 #pragma unroll
-  for (int k = 0; k < PS; ++k) {
-    int q = tid + k * NT;
-    sidx[k] = -1;
-    if (q < nper) {
-      int pr, pc;
-      bool outer;
-      if (q < w + 4) { pr = R0; pc = C0 + q; outer = true; }
-      else if ((q -= w + 4) < w + 4) { pr = R1; pc = C0 + q; outer = true; }
-      else if ((q -= w + 4) < h + 2) { pr = R0 + 1 + q; pc = C0; outer = true; }
-      else if ((q -= h + 2) < h + 2) { pr = R0 + 1 + q; pc = C1; outer = true; }
-      else if ((q -= h + 2) < w + 2) { pr = R0 + 1; pc = C0 + 1 + q; outer = false; }
-      else if ((q -= w + 2) < w + 2) { pr = R1 - 1; pc = C0 + 1 + q; outer = false; }
-      else if ((q -= w + 2) < h) { pr = R0 + 2 + q; pc = C0 + 1; outer = false; }
-      else { q -= h; pr = R0 + 2 + q; pc = C1 - 1; outer = false; }
-      const int idx = pr * RS + pc;
-      sidx[k] = idx;
+    for (int k = 0; k < PS; ++k) {
+      int q = tid + k * NT;
+      sidx[k] = -1;
+      if (q < nper) {
+        int pr, pc;
+        bool outer;
+        if (q < w + 4) { pr = R0; pc = C0 + q; outer = true; }
+        else if ((q -= w + 4) < w + 4) { pr = R1; pc = C0 + q; outer = true; }
+        else if ((q -= w + 4) < h + 2) { pr = R0 + 1 + q; pc = C0; outer = true; }
+        else if ((q -= h + 2) < h + 2) { pr = R0 + 1 + q; pc = C1; outer = true; }
+        else if ((q -= h + 2) < w + 2) { pr = R0 + 1; pc = C0 + 1 + q; outer = false; }
+        else if ((q -= w + 2) < w + 2) { pr = R1 - 1; pc = C0 + 1 + q; outer = false; }
+        else if ((q -= w + 2) < h) { pr = R0 + 2 + q; pc = C0 + 1; outer = false; }
+        else { q -= h; pr = R0 + 2 + q; pc = C1 - 1; outer = false; }
+        const int idx = pr * RS + pc;
+        sidx[k] = idx;
 #pragma unroll
-      for (int s = 0; s < 3; ++s) sv[k][s] = sub[s * PV + idx];
-      if (outer) {
+        for (int s = 0; s < 3; ++s) sv[k][s] = sub[s * PV + idx];
+        if (outer) {
 #pragma unroll
-        for (int s = 0; s < 3; ++s) sub[s * PV + idx] = 0.f;
-      } else if (masked(pr - r0, pc - c0, hm, wm, d4)) {
+          for (int s = 0; s < 3; ++s) sub[s * PV + idx] = 0.f;
+        } else if (masked(pr - r0, pc - c0, hm, wm, d4)) {
+#pragma unroll
+          for (int s = 0; s < 3; ++s) sub[s * PV + idx] = p.ext[s];
+        }
+      }
+    }
+    for (int q = tid; q < h * w; q += NT) {
+      const int i = 1 + q / w, j = 1 + q % w;
+      if (masked(i, j, hm, wm, d4)) {
+        const int idx = (r0 + i) * RS + (c0 + j);
 #pragma unroll
         for (int s = 0; s < 3; ++s) sub[s * PV + idx] = p.ext[s];
       }
     }
   }
-  for (int q = tid; q < h * w; q += NT) {
-    const int i = 1 + q / w, j = 1 + q % w;
-    if (masked(i, j, hm, wm, d4)) {
-      const int idx = (r0 + i) * RS + (c0 + j);
+
+  __device__ void restore(float *sub, int PV) const {
 #pragma unroll
-      for (int s = 0; s < 3; ++s) sub[s * PV + idx] = p.ext[s];
+    for (int k = 0; k < PS; ++k) {
+      if (sidx[k] >= 0) {
+#pragma unroll
+        for (int s = 0; s < 3; ++s) sub[s * PV + sidx[k]] = sv[k][s];
+      }
     }
   }
+};
+
+template <int NT, int L, int KMAX, int RS, int PS, bool LANES = false>
+__device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5],
+                        const uint32_t jac[KMAX], const uint32_t fin[KMAX], uint32_t &updates) {
+  const int r0 = roi[0], c0 = roi[1], h = roi[2], w = roi[3];
+  if (h == 0 || w == 0) return;
+  TempGrid<NT, RS, PS> tg;
+  tg.setup(p, sub, PV, roi);
   const int S = p.substeps;
   const int total = S > 1 ? S : 1;
   {
@@ -530,13 +554,7 @@ __device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5]
     }
     __syncthreads();
   }
-#pragma unroll
-  for (int k = 0; k < PS; ++k) {
-    if (sidx[k] >= 0) {
-#pragma unroll
-      for (int s = 0; s < 3; ++s) sub[s * PV + sidx[k]] = sv[k][s];
-    }
-  }
+  tg.restore(sub, PV);
   __syncthreads();
 }
 
@@ -686,8 +704,7 @@ struct LaneRuns {
   static constexpr int kSlots(int L) {
     return (kMaxLanes(L) + NT - 1) / NT < KCAP ? (kMaxLanes(L) + NT - 1) / NT : KCAP;
   }
-  static constexpr int kSlotsMax = kSlots(3) > kSlots(6) ? kSlots(3) : kSlots(6);
-  static_assert(KCAP * NT >= kMaxLanes(6), "height-6 layouts must fit");
+  static constexpr int kSlotsUpTo(int L) { return L <= 0 ? 0 : (kSlots(L) > kSlotsUpTo(L - 1) ? kSlots(L) : kSlotsUpTo(L - 1)); }
 
   struct Lane {
     int run, col;
@@ -726,11 +743,15 @@ struct LaneRuns {
   }
 };
 
-template <int NT, int RS, int PS, int KCAP>
+// Band heights H1 < H2 < H3 are tried per ROI; layouts of height H3 always
+// fit the KCAP lane slots.
+template <int NT, int RS, int PS, int KCAP, int H1 = 3, int H2 = 4, int H3 = 6,
+          int POLICY = 0>
 struct LaneRunDiffuser {
   using LR = LaneRuns<NT, RS, KCAP>;
+  static_assert(KCAP * NT >= LR::kMaxLanes(H3), "the tallest bands must fit");
   static constexpr int kPlanes = 3;
-  static constexpr int KM = LR::kSlotsMax;
+  static constexpr int KM = LR::kSlotsUpTo(H3);
   uint32_t jac[KM], fin[KM];
   int L = 4;
 
@@ -755,7 +776,7 @@ struct LaneRunDiffuser {
       chord[i] = e > a ? (a | (e << 16)) : 0;
     }
     __syncthreads();
-    int Lc = 6;
+    int Lc = H3;
     auto jac_runs = [&](int b, int &lo, int &hi) {
       int a = wm, e = 0;
       for (int l = 0; l < Lc && b * Lc + l < hm; ++l) {
@@ -767,17 +788,18 @@ struct LaneRunDiffuser {
     };
     auto fin_runs = [&](int, int &lo, int &hi) { lo = 1; hi = w + 1; };
     typename LR::Lane dummy;
-    int best = 6, best_cost = -1;
+    int best = H3, best_cost = -1;
     {
-      const int lcs[3] = {3, 4, 6};
-      const int slots[3] = {LR::kSlots(3), LR::kSlots(4), LR::kSlots(6)};
+      const int lcs[3] = {H1, H2, H3};
+      const int slots[3] = {LR::kSlots(H1), LR::kSlots(H2), LR::kSlots(H3)};
 #pragma unroll
       for (int ci = 0; ci < 3; ++ci) {
         Lc = lcs[ci];
         const int n = LR::layout((hm + Lc - 1) / Lc, jac_runs, -1, dummy);
         const int nf = LR::layout((h + Lc - 1) / Lc, fin_runs, -1, dummy);
         if (n > slots[ci] * NT || nf > slots[ci] * NT) continue;
-        const int cost = ((n + 63) / 64) * (24 * Lc + 30);
+        // POLICY 0: fewest wave instructions; 1: most waves (lowest bands).
+        const int cost = POLICY == 0 ? ((n + 63) / 64) * (24 * Lc + 30) : Lc;
         if (best_cost < 0 || cost < best_cost) { best = Lc; best_cost = cost; }
       }
     }
@@ -819,9 +841,271 @@ struct LaneRunDiffuser {
   }
 
   __device__ void run(const DevParams &p, float *sub, int PV, const int roi[5], uint32_t &upd) {
-    if (L == 3) run_l<3>(p, sub, PV, roi, upd);
-    else if (L == 4) run_l<4>(p, sub, PV, roi, upd);
-    else run_l<6>(p, sub, PV, roi, upd);
+    if (L == H1) run_l<H1>(p, sub, PV, roi, upd);
+    else if (L == H2) run_l<H2>(p, sub, PV, roi, upd);
+    else run_l<H3>(p, sub, PV, roi, upd);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Pair-run layout: each lane holds two adjacent columns (ja, ja + 1) of one
+// band, ja + padded column even, so a row of both is one 8-byte LDS read.
+// The inner neighbour of each column is the lane's other column; the outer
+// one comes from the adjacent lane through DPP.  A band's run of columns
+// with unmasked sites, widened by one column on each side, becomes the
+// pairs covering it, placed on consecutive lanes of one wave; a run cut at
+// a wave boundary repeats the pair at the cut.  A column is written only by
+// a lane whose neighbouring lane on that column's outer side belongs to the
+// same run piece (so the first lane of a piece never writes ja, the last
+// never writes ja + 1).
+// Item word: ja + 2 (bits 0-7), first band row (8-15, 0xff: no item), two
+// bits per row l (16 + 2l: write ja, 17 + 2l: write ja + 1).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kNoPair = 0xff00u;
+
+template <int NT, int RS, int KCAP>
+struct PairRuns {
+  static constexpr int kBands(int L) { return (RS - 2 + L - 1) / L; }
+  // Pairs a height-L layout can need: per band at most RS / 2 + 2, one
+  // repeated pair per wave cut, one lane lost per wave.
+  static constexpr int kMaxLanes(int L) {
+    return kBands(L) * (RS / 2 + 2) + 2 * (kBands(L) * (RS / 2 + 2) / 62 + 2);
+  }
+  static constexpr int kSlots(int L) {
+    return (kMaxLanes(L) + NT - 1) / NT < KCAP ? (kMaxLanes(L) + NT - 1) / NT : KCAP;
+  }
+
+  struct Lane {
+    int run, ja;
+    bool own_a, own_b;
+  };
+
+  // Runs r = 0..nr-1 have owner columns [lo, hi) (runfn); e = parity of the
+  // padded column of mask-box column 0.  Returns the lanes used; for lane
+  // q >= 0 fills `out` (run -1: no item).
+  template <typename RunFn>
+  __device__ static int layout(int nr, const RunFn &runfn, int e, int q, Lane &out) {
+    out.run = -1;
+    int pos = 0;
+    for (int r = 0; r < nr; ++r) {
+      int lo, hi;
+      runfn(r, lo, hi);
+      if (hi <= lo) continue;
+      // pair p covers columns 2p - e and 2p + 1 - e
+      const int p0 = (lo - 1 + e + 2) / 2 - 1, p1 = (hi + e) / 2;
+      int c = p0;
+      for (;;) {
+        const int room = 64 - (pos & 63);
+        if (room < 2) {
+          pos += room;
+          continue;
+        }
+        const int take = min(room, p1 + 1 - c);
+        if (q >= pos && q < pos + take) {
+          const int t = q - pos, ja = 2 * (c + t) - e;
+          out.run = r;
+          out.ja = ja;
+          out.own_a = t > 0 && ja >= lo && ja < hi;
+          out.own_b = t < take - 1 && ja + 1 >= lo && ja + 1 < hi;
+        }
+        pos += take;
+        if (c + take - 1 >= p1) break;
+        c += take - 1;
+      }
+    }
+    return pos;
+  }
+};
+
+template <int NT, int L, int KMAX, int RS, int PS>
+__device__ void diffuse_pairs(const DevParams &p, float *sub, int PV, const int roi[5],
+                              const uint32_t jac[KMAX], const uint32_t fin[KMAX],
+                              uint32_t &updates) {
+  const int r0 = roi[0], c0 = roi[1], h = roi[2], w = roi[3];
+  if (h == 0 || w == 0) return;
+  TempGrid<NT, RS, PS> tg;
+  tg.setup(p, sub, PV, roi);
+  const int S = p.substeps;
+  const int total = S > 1 ? S : 1;
+  {
+    uint32_t na = 0, nf = 0;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      na += __popc(jac[k] >> 16);
+      nf += __popc(fin[k] >> 16);
+    }
+    updates += na * (uint32_t)(total - 1) + nf;
+  }
+  const float R2 = (float)0.70710678118654752440;
+  const float F = (float)(4.0 + (double)(float)(2.0 * 1.41421356237309504880));
+  const int origin = r0 * RS + c0;
+  __syncthreads();
+  for (int step = 0; step < total; ++step) {
+    const bool final_sub = (step == total - 1);
+    float ra[KMAX][L][3], rb[KMAX][L][3];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      const uint32_t it = final_sub ? fin[k] : jac[k];
+      if ((it & 0xff00u) == kNoPair) continue;
+      const int ja = (int)(it & 0xffu) - 2, i0 = (int)((it >> 8) & 0xffu);
+      const int a0 = origin + (i0 - 1) * RS + ja;
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const float2 *P = reinterpret_cast<const float2 *>(sub + s * PV + a0);
+        const float kk = p.k[s];
+        const float2 u = P[0], m = P[RS / 2];
+        // Centre values and horizontal sums of the rows above, at and below.
+        float uca = u.x, ucb = u.y, mca = m.x, mcb = m.y;
+        float hua = from_left_lane(u.y) + u.y, hub = u.x + from_right_lane(u.x);
+        float hma = from_left_lane(m.y) + m.y, hmb = m.x + from_right_lane(m.x);
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+          const float2 d = P[(l + 2) * (RS / 2)];
+          const float hda = from_left_lane(d.y) + d.y, hdb = d.x + from_right_lane(d.x);
+          const float oa = (uca + d.x) + hma, ob = (ucb + d.y) + hmb;
+          const float dda = hua + hda, ddb = hub + hdb;
+          float La = fmaf(dda, R2, oa), Lb = fmaf(ddb, R2, ob);
+          La = fmaf(-F, mca, La);
+          Lb = fmaf(-F, mcb, Lb);
+          ra[k][l][s] = fmaf(kk, La, mca);
+          rb[k][l][s] = fmaf(kk, Lb, mcb);
+          uca = mca; ucb = mcb; hua = hma; hub = hmb;
+          mca = d.x; mcb = d.y; hma = hda; hmb = hdb;
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      const uint32_t it = final_sub ? fin[k] : jac[k];
+      const uint32_t m = it >> 16;
+      if (m == 0u) continue;
+      const int ja = (int)(it & 0xffu) - 2, i0 = (int)((it >> 8) & 0xffu);
+      float *P = sub + origin + i0 * RS + ja;
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        const uint32_t two = (m >> (2 * l)) & 3u;
+        if (two == 3u) {
+#pragma unroll
+          for (int s = 0; s < 3; ++s)
+            *reinterpret_cast<float2 *>(P + s * PV + l * RS) = make_float2(ra[k][l][s], rb[k][l][s]);
+        } else if (two == 1u) {
+#pragma unroll
+          for (int s = 0; s < 3; ++s) P[s * PV + l * RS] = ra[k][l][s];
+        } else if (two == 2u) {
+#pragma unroll
+          for (int s = 0; s < 3; ++s) P[s * PV + l * RS + 1] = rb[k][l][s];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  tg.restore(sub, PV);
+  __syncthreads();
+}
+
+// Multi-band diffusion in the pair-run layout; band heights H1 < H2 < H3
+// tried per ROI, H3 always fitting KCAP slots per thread.
+template <int NT, int RS, int PS, int KCAP, int H1 = 4, int H2 = 6, int H3 = 8,
+          int POLICY = 0>
+struct PairRunDiffuser {
+  using PR = PairRuns<NT, RS, KCAP>;
+  static_assert(KCAP * NT >= PR::kMaxLanes(H3), "the tallest bands must fit");
+  static_assert(H3 <= 8, "two mask bits per row in 16 bits");
+  static_assert(RS % 2 == 0, "pairs need an even row stride");
+  static constexpr int kPlanes = 3;
+  static constexpr int KM = KCAP;
+  uint32_t jac[KM], fin[KM];
+  int L = H3;
+
+  __device__ void set_sim(int) {}
+
+  __device__ void plan(const int roi[5], Shared &sm) {
+    const int c0 = roi[1], h = roi[2], w = roi[3], dist = roi[4];
+#pragma unroll
+    for (int k = 0; k < KM; ++k) { jac[k] = kNoPair; fin[k] = kNoPair; }
+    if (h == 0 || w == 0) return;
+    const int hm = h + 2, wm = w + 2, d4 = 4 * dist * dist;
+    const int e = c0 & 1;  // parity of the padded column of mask-box column 0
+    int32_t *chord = sm.red;  // per mask-box row: first | (past-last << 16)
+    for (int i = threadIdx.x; i < hm; i += NT) {
+      const int ai = 2 * (i + 1) - hm;
+      const int rem = d4 - ai * ai;
+      int a = wm, en = 0;
+      if (rem > 0)
+        for (int j = 0; j < wm; ++j) {
+          const int bj = 2 * (j + 1) - wm;
+          if (bj * bj < rem) { a = min(a, j); en = max(en, j + 1); }
+        }
+      chord[i] = en > a ? (a | (en << 16)) : 0;
+    }
+    __syncthreads();
+    int Lc = H3;
+    auto jac_runs = [&](int b, int &lo, int &hi) {
+      int a = wm, en = 0;
+      for (int l = 0; l < Lc && b * Lc + l < hm; ++l) {
+        const int c = chord[b * Lc + l];
+        if (c) { a = min(a, c & 0xffff); en = max(en, c >> 16); }
+      }
+      lo = a;
+      hi = en > a ? en : a;
+    };
+    auto fin_runs = [&](int, int &lo, int &hi) { lo = 1; hi = w + 1; };
+    typename PR::Lane dummy;
+    int best = H3, best_cost = -1;
+    {
+      const int lcs[3] = {H1, H2, H3};
+#pragma unroll
+      for (int ci = 0; ci < 3; ++ci) {
+        Lc = lcs[ci];
+        const int n = PR::layout((hm + Lc - 1) / Lc, jac_runs, e, -1, dummy);
+        const int nf = PR::layout((h + Lc - 1) / Lc, fin_runs, e, -1, dummy);
+        if (n > KCAP * NT || nf > KCAP * NT) continue;
+        // POLICY 0: fewest wave instructions; 1: most waves (lowest bands).
+        const int cost = POLICY == 0 ? ((n + 63) / 64) * (42 * Lc + 40) : Lc;
+        if (best_cost < 0 || cost < best_cost) { best = Lc; best_cost = cost; }
+      }
+    }
+    L = Lc = best;
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      typename PR::Lane ln;
+      PR::layout((hm + L - 1) / L, jac_runs, e, (int)threadIdx.x + k * NT, ln);
+      if (ln.run >= 0) {
+        const int i0 = ln.run * L;
+        uint32_t m = 0u;
+        for (int l = 0; l < L; ++l) {
+          const int i = i0 + l;
+          if (i >= hm) break;
+          if (ln.own_a && !masked(i, ln.ja, hm, wm, d4)) m |= 1u << (2 * l);
+          if (ln.own_b && !masked(i, ln.ja + 1, hm, wm, d4)) m |= 2u << (2 * l);
+        }
+        jac[k] = (uint32_t)(ln.ja + 2) | ((uint32_t)i0 << 8) | (m << 16);
+      }
+      PR::layout((h + L - 1) / L, fin_runs, e, (int)threadIdx.x + k * NT, ln);
+      if (ln.run >= 0) {
+        const int i0 = 1 + ln.run * L;
+        uint32_t m = 0u;
+        for (int l = 0; l < L; ++l) {
+          if (i0 + l > h) break;
+          if (ln.own_a) m |= 1u << (2 * l);
+          if (ln.own_b) m |= 2u << (2 * l);
+        }
+        fin[k] = (uint32_t)(ln.ja + 2) | ((uint32_t)i0 << 8) | (m << 16);
+      }
+    }
+    __syncthreads();
+  }
+
+  template <int LL>
+  __device__ void run_l(const DevParams &p, float *sub, int PV, const int roi[5], uint32_t &upd) {
+    diffuse_pairs<NT, LL, KM, RS, PS>(p, sub, PV, roi, jac, fin, upd);
+  }
+
+  __device__ void run(const DevParams &p, float *sub, int PV, const int roi[5], uint32_t &upd) {
+    if (L == H1) run_l<H1>(p, sub, PV, roi, upd);
+    else if (L == H2) run_l<H2>(p, sub, PV, roi, upd);
+    else run_l<H3>(p, sub, PV, roi, upd);
   }
 };
 
