@@ -58,8 +58,9 @@ class Batch {
   double last_ms() const { return last_ms_; }
   int64_t last_launches() const { return last_launches_; }
   // Kernel variant for lattices up to hmax x wmax: among the fitting
-  // variants of `kind` (-1: any whose persistent kernel fits in LDS), the
-  // smallest LDS row stride, then the ping-pong scheme, then table order.
+  // variants of `kind` (-1: any whose persistent kernel fits in LDS) with
+  // the smallest LDS row stride, the preferred one (kind -1: the fastest
+  // measured), else the highest kind, then table order.
   static int choose_variant(int hmax, int wmax, int kind = -1);
   // Preferred variant per LDS row stride class (-1: choose_variant).
   void set_preferred_variant(int rs, int variant);

@@ -361,7 +361,7 @@ hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds
   X(21, 3, 512, 2, 0, 1, 56, 2)   \
   X(22, 3, 256, 3, 0, 2, 56, 2)   \
   X(23, 3, 256, 1, 0, 2, 36, 4)   \
-  X(24, 3, 512, 3, 0, 1, 88, 2)   \
+  X(24, 3, 512, 3, 0, 2, 88, 2)   \
   X(25, 3, 256, 2, 1, 2, 56, 4)   \
   X(26, 3, 256, 2, 1, 2, 56, 3)   \
   X(27, 4, 256, 1, 0, 2, 56, 4)   \
@@ -375,7 +375,10 @@ hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds
   X(35, 3, 256, 4, 3, 2, 56, 4)   \
   X(36, 3, 256, 3, 4, 2, 56, 4)   \
   X(37, 3, 512, 1, 2, 1, 56, 4)   \
-  X(38, 3, 256, 2, 2, 2, 56, 3)
+  X(38, 3, 256, 2, 2, 2, 56, 3)   \
+  X(39, 3, 512, 3, 4, 2, 88, 2)   \
+  X(40, 3, 512, 3, 2, 2, 88, 2)   \
+  X(41, 3, 256, 2, 4, 2, 36, 4)
 
 const VariantInfo *variant_table(int *count) {
   static const VariantInfo table[] = {
