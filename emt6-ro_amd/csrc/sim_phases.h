@@ -382,12 +382,12 @@ __device__ void plan_diffusion(const int roi[5], uint32_t jac[KMAX], uint32_t fi
 }
 
 // Horizontal neighbours through the wave: DPP wave_shr:1 gives lane i the
-// value of lane i - 1, wave_shl:1 that of lane i + 1.
+// value of lane i - 1, wave_shl:1 that of lane i + 1 (lanes 0 and 63 get 0).
 __device__ __forceinline__ float from_left_lane(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float from_right_lane(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 
 // Item word of the lane-run layout (LaneRunDiffuser): column + 1 (bits 0-7),
