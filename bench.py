@@ -32,7 +32,7 @@ FP32_NONPACKED_TFLOPS = 78.6  # the same without packed FP32 (not emitted by thi
 # HBM bytes per launch of the persistent kernel on this workload (one hour of
 # 20,000 simulations): FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy, from
 # profiles/persistent_kernel_batched_uniform_53x53.md.
-TRAFFIC_BYTES_PER_LAUNCH = 4.73e9 + 24.32e9
+TRAFFIC_BYTES_PER_LAUNCH = 3.21e9 + 5.37e9
 
 
 def parse():
