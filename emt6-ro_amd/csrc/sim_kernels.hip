@@ -104,9 +104,14 @@ __global__ __launch_bounds__(NT, WPE) void persistent_kernel(RunArgs a) {
       if (t % 128u == 0u) rebuild_list<NT>(st, H, W, slots, d.cap, pfx, sm);
       if (t % 32u == 0u) {
         find_roi<NT>(st, H, W, sm);
+        // The plan depends on the ROI only; most refreshes find it unchanged.
+        bool same = true;
 #pragma unroll
-        for (int q = 0; q < 5; ++q) roi[q] = sm.roi[q];
-        diff.plan(roi, sm);
+        for (int q = 0; q < 5; ++q) {
+          same = same && roi[q] == sm.roi[q];
+          roi[q] = sm.roi[q];
+        }
+        if (!same) diff.plan(roi, sm);
       }
       diff.run(a.p, sub, PV, roi, updates);
       const float dose = dose_at(a, d.prot, t);
