@@ -29,10 +29,10 @@ METRIC = "simulated tumour-hours/sec on 1 MI355X; GA evals/sec surrogate vs plai
 FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 peak
 FLOP_PER_SITE_SUBSTEP = 36    # 3 substrates x (6 adds + 3 fma x 2)
 FP32_NONPACKED_TFLOPS = 78.6  # the same without packed FP32 (not emitted by this build)
-# HBM bytes per launch of the persistent kernel on this workload (one hour of
-# 20,000 simulations): FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy, from
-# profiles/persistent_kernel_batched_uniform_53x53.md.
-TRAFFIC_BYTES_PER_LAUNCH = 3.21e9 + 5.37e9
+# HBM bytes per launch of the persistent kernel (variant 36) on this workload
+# (one hour of 20,000 simulations): rocprofv3 FETCH_SIZE + WRITE_SIZE, each
+# scaled by what the counter reads for a 1 GiB device copy (x2.0, x1.0).
+TRAFFIC_BYTES_PER_LAUNCH = 3.19e9 + 5.88e9
 
 
 def parse():
