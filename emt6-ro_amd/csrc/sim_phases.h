@@ -397,24 +397,25 @@ __device__ __forceinline__ uint32_t pack_lane_item(int j, int i0, uint32_t mask)
   return (uint32_t)(j + 1) | ((uint32_t)i0 << 8) | (mask << 16) | kItemValid;
 }
 
-// One diffusion step on the padded lattice in LDS (planes PV floats apart,
-// row stride RS).  `jac` and `fin` come from the plan for the same ROI.
-// LANES = false: items are columns of bands and read their horizontal
-// neighbours from LDS (plan_diffusion / MultiBand layout).  LANES = true:
-// items are laid out in runs of consecutive lanes holding consecutive
-// columns of one band, with a read-only item at each end of a run (mask 0),
-// so every written site finds its left and right neighbours in the adjacent
-// lanes and reads only its own column from LDS.
 // The reference's temporary grid (diffusion.cu:150-162) made in place: the
 // two rings around the ROI are saved to registers and set to their
 // temporary values (outer ring 0, masked sites the external levels), masked
 // ROI sites get the external levels, and restore() puts the rings back.
 template <int NT, int RS, int PS>
 struct TempGrid {
+  // Masked ROI sites a thread keeps between ROI changes (16-bit plane
+  // indices, two per word); a ROI with more on some thread uses the loop.
+  static constexpr int kMasked = 4;
+  static constexpr uint32_t kOuter = 1u << 16, kMask = 1u << 17, kNone = 0xffffffffu;
   float sv[PS][3];
-  int sidx[PS];
+  uint32_t ring[PS];           // plane index | kOuter | kMask, or kNone
+  uint32_t mk[kMasked / 2];    // masked ROI sites, 0xffff: none
+  bool mk_all;                 // every thread's masked sites are in mk
 
-  __device__ void setup(const DevParams &p, float *sub, int PV, const int roi[5]) {
+  // The ring sites and masked ROI sites of `roi`; they depend on the ROI
+  // only, so callers that keep the TempGrid across steps call this when the
+  // ROI changes and apply() every step.
+  __device__ void prepare(const int roi[5], Shared &sm) {
     const int r0 = roi[0], c0 = roi[1], h = roi[2], w = roi[3], dist = roi[4];
     const int tid = threadIdx.x;
     const int hm = h + 2, wm = w + 2;
@@ -425,7 +426,108 @@ struct TempGrid {
 #pragma unroll
     for (int k = 0; k < PS; ++k) {
       int q = tid + k * NT;
-      sidx[k] = -1;
+      ring[k] = kNone;
+      if (q < nper && h > 0 && w > 0) {
+        int pr, pc;
+        bool outer;
+        if (q < w + 4) { pr = R0; pc = C0 + q; outer = true; }
+        else if ((q -= w + 4) < w + 4) { pr = R1; pc = C0 + q; outer = true; }
+        else if ((q -= w + 4) < h + 2) { pr = R0 + 1 + q; pc = C0; outer = true; }
+        else if ((q -= h + 2) < h + 2) { pr = R0 + 1 + q; pc = C1; outer = true; }
+        else if ((q -= h + 2) < w + 2) { pr = R0 + 1; pc = C0 + 1 + q; outer = false; }
+        else if ((q -= w + 2) < w + 2) { pr = R1 - 1; pc = C0 + 1 + q; outer = false; }
+        else if ((q -= w + 2) < h) { pr = R0 + 2 + q; pc = C0 + 1; outer = false; }
+        else { q -= h; pr = R0 + 2 + q; pc = C1 - 1; outer = false; }
+        ring[k] = (uint32_t)(pr * RS + pc) |
+                  (outer ? kOuter : masked(pr - r0, pc - c0, hm, wm, d4) ? kMask : 0u);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kMasked / 2; ++k) mk[k] = 0xffffffffu;
+    int n = 0;
+    for (int q = tid; q < h * w; q += NT) {
+      const int i = 1 + q / w, j = 1 + q % w;
+      if (masked(i, j, hm, wm, d4)) {
+        const uint32_t idx = (uint32_t)((r0 + i) * RS + (c0 + j));
+#pragma unroll
+        for (int k = 0; k < kMasked; ++k)
+          if (k == n) {
+            const int sh = 16 * (k & 1);
+            mk[k >> 1] = (mk[k >> 1] & ~(0xffffu << sh)) | (idx << sh);
+          }
+        ++n;
+      }
+    }
+    // Workgroup AND of (n <= kMasked) in the last reduction word, which
+    // the chord table of a plan (at most 88 rows) leaves alone.
+    int32_t &all = sm.red[kRedInts - 1];
+    if (tid == 0) all = 1;
+    __syncthreads();
+    if (n > kMasked) all = 0;
+    __syncthreads();
+    mk_all = all != 0;
+    __syncthreads();
+  }
+
+  // The reference's temporary grid (diffusion.cu:150-162) made in place:
+  // ring values saved, outer ring 0, masked ring and ROI sites the external
+  // levels.
+  __device__ void apply(const DevParams &p, float *sub, int PV, const int roi[5]) {
+#pragma unroll
+    for (int k = 0; k < PS; ++k) {
+      if (ring[k] != kNone) {
+        const int idx = (int)(ring[k] & 0xffffu);
+#pragma unroll
+        for (int s = 0; s < 3; ++s) sv[k][s] = sub[s * PV + idx];
+        if (ring[k] & kOuter) {
+#pragma unroll
+          for (int s = 0; s < 3; ++s) sub[s * PV + idx] = 0.f;
+        } else if (ring[k] & kMask) {
+#pragma unroll
+          for (int s = 0; s < 3; ++s) sub[s * PV + idx] = p.ext[s];
+        }
+      }
+    }
+    if (mk_all) {
+#pragma unroll
+      for (int k = 0; k < kMasked; ++k) {
+        const uint32_t idx = (mk[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+        if (idx != 0xffffu) {
+#pragma unroll
+          for (int s = 0; s < 3; ++s) sub[s * PV + idx] = p.ext[s];
+        }
+      }
+    } else {
+      masked_loop(p, sub, PV, roi);
+    }
+  }
+
+  __device__ static void masked_loop(const DevParams &p, float *sub, int PV, const int roi[5]) {
+    const int r0 = roi[0], c0 = roi[1], h = roi[2], w = roi[3], dist = roi[4];
+    const int hm = h + 2, wm = w + 2, d4 = 4 * dist * dist;
+    for (int q = threadIdx.x; q < h * w; q += NT) {
+      const int i = 1 + q / w, j = 1 + q % w;
+      if (masked(i, j, hm, wm, d4)) {
+        const int idx = (r0 + i) * RS + (c0 + j);
+#pragma unroll
+        for (int s = 0; s < 3; ++s) sub[s * PV + idx] = p.ext[s];
+      }
+    }
+  }
+
+  // Per-step form for callers that do not keep the TempGrid: prepare()'s
+  // ring decoding, then the masked ROI sites by the loop.
+  __device__ void setup(const DevParams &p, float *sub, int PV, const int roi[5]) {
+    const int r0 = roi[0], c0 = roi[1], h = roi[2], w = roi[3], dist = roi[4];
+    const int tid = threadIdx.x;
+    const int hm = h + 2, wm = w + 2;
+    const int d4 = 4 * dist * dist;
+    const int R0 = r0 - 1, R1 = r0 + h + 2, C0 = c0 - 1, C1 = c0 + w + 2;
+    const int nper = 4 * w + 4 * h + 16;
+#pragma unroll
+    for (int k = 0; k < PS; ++k) {
+      int q = tid + k * NT;
+      ring[k] = kNone;
       if (q < nper) {
         int pr, pc;
         bool outer;
@@ -438,7 +540,7 @@ struct TempGrid {
         else if ((q -= w + 2) < h) { pr = R0 + 2 + q; pc = C0 + 1; outer = false; }
         else { q -= h; pr = R0 + 2 + q; pc = C1 - 1; outer = false; }
         const int idx = pr * RS + pc;
-        sidx[k] = idx;
+        ring[k] = (uint32_t)idx;
 #pragma unroll
         for (int s = 0; s < 3; ++s) sv[k][s] = sub[s * PV + idx];
         if (outer) {
@@ -450,34 +552,40 @@ struct TempGrid {
         }
       }
     }
-    for (int q = tid; q < h * w; q += NT) {
-      const int i = 1 + q / w, j = 1 + q % w;
-      if (masked(i, j, hm, wm, d4)) {
-        const int idx = (r0 + i) * RS + (c0 + j);
-#pragma unroll
-        for (int s = 0; s < 3; ++s) sub[s * PV + idx] = p.ext[s];
-      }
-    }
+    masked_loop(p, sub, PV, roi);
   }
 
   __device__ void restore(float *sub, int PV) const {
 #pragma unroll
     for (int k = 0; k < PS; ++k) {
-      if (sidx[k] >= 0) {
+      if (ring[k] != kNone) {
+        const int idx = (int)(ring[k] & 0xffffu);
 #pragma unroll
-        for (int s = 0; s < 3; ++s) sub[s * PV + sidx[k]] = sv[k][s];
+        for (int s = 0; s < 3; ++s) sub[s * PV + idx] = sv[k][s];
       }
     }
   }
 };
 
+// One diffusion step on the padded lattice in LDS (planes PV floats apart,
+// row stride RS).  `jac` and `fin` come from the plan for the same ROI.
+// LANES = false: items are columns of bands and read their horizontal
+// neighbours from LDS (plan_diffusion / MultiBand layout).  LANES = true:
+// items are laid out in runs of consecutive lanes holding consecutive
+// columns of one band, with a read-only item at each end of a run (mask 0),
+// so every written site finds its left and right neighbours in the adjacent
+// lanes and reads only its own column from LDS.
+// `kept`: a TempGrid the caller prepared for this ROI and keeps between
+// steps; without it one is set up for this step.
 template <int NT, int L, int KMAX, int RS, int PS, bool LANES = false>
 __device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5],
-                        const uint32_t jac[KMAX], const uint32_t fin[KMAX], uint32_t &updates) {
+                        const uint32_t jac[KMAX], const uint32_t fin[KMAX], uint32_t &updates,
+                        TempGrid<NT, RS, PS> *kept = nullptr) {
   const int r0 = roi[0], c0 = roi[1], h = roi[2], w = roi[3];
   if (h == 0 || w == 0) return;
-  TempGrid<NT, RS, PS> tg;
-  tg.setup(p, sub, PV, roi);
+  TempGrid<NT, RS, PS> local;
+  TempGrid<NT, RS, PS> &tg = kept ? *kept : local;
+  if (kept) tg.apply(p, sub, PV, roi); else tg.setup(p, sub, PV, roi);
   const int S = p.substeps;
   const int total = S > 1 ? S : 1;
   {
@@ -750,15 +858,18 @@ template <int NT, int RS, int PS, int KCAP, int H1 = 3, int H2 = 4, int H3 = 6,
 struct LaneRunDiffuser {
   using LR = LaneRuns<NT, RS, KCAP>;
   static_assert(KCAP * NT >= LR::kMaxLanes(H3), "the tallest bands must fit");
+  static_assert(RS < kRedInts, "the chord table and TempGrid's flag share sm.red");
   static constexpr int kPlanes = 3;
   static constexpr int KM = LR::kSlotsUpTo(H3);
   uint32_t jac[KM], fin[KM];
   int L = 4;
+  TempGrid<NT, RS, PS> tg;   // kept between ROI changes
 
   __device__ void set_sim(int) {}
 
   __device__ void plan(const int roi[5], Shared &sm) {
     const int h = roi[2], w = roi[3], dist = roi[4];
+    tg.prepare(roi, sm);
 #pragma unroll
     for (int k = 0; k < KM; ++k) { jac[k] = 0u; fin[k] = 0u; }
     if (h == 0 || w == 0) return;
@@ -837,7 +948,7 @@ struct LaneRunDiffuser {
     uint32_t j2[K], f2[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) { j2[k] = jac[k]; f2[k] = fin[k]; }
-    diffuse<NT, LL, K, RS, PS, true>(p, sub, PV, roi, j2, f2, upd);
+    diffuse<NT, LL, K, RS, PS, true>(p, sub, PV, roi, j2, f2, upd, &tg);
   }
 
   __device__ void run(const DevParams &p, float *sub, int PV, const int roi[5], uint32_t &upd) {
