@@ -100,8 +100,9 @@ def cpu_baseline(params, wl, n_sims, hours):
 
 
 def surrogate_sample(train_steps=200, batch=4096, n_data=200000, n_screen=100000):
-    """The fitness surrogate (152-512-512-512-1, BF16 on the matrix cores):
-    Adam steps on random data and one screening pass, timed on the host."""
+    """The fitness surrogate's network (N_FEATURES-512-512-512-1, BF16 on the
+    matrix cores): Adam steps on random data and one screening pass, timed
+    on the host."""
     import numpy as np
     from emt6ro.surrogate import MLP, N_FEATURES
     hidden = 512
@@ -119,7 +120,7 @@ def surrogate_sample(train_steps=200, batch=4096, n_data=200000, n_screen=100000
     t0 = time.perf_counter()
     net.predict(Xs)
     screen_s = time.perf_counter() - t0
-    # GEMM FLOP per sample: padded input 160 columns (K padded to 32).
+    # GEMM FLOP per sample, with the input padded to a multiple of 32 columns.
     kin = (N_FEATURES + 31) // 32 * 32
     fwd = 2 * (kin * hidden + 2 * hidden * hidden) * batch
     # forward, weight gradients (same shapes), input gradients of layers 2-3

@@ -126,12 +126,20 @@ def _spearman(a, b):
     return float(np.corrcoef(ra, rb)[0, 1])
 
 
+def _rows(model, g, descriptors):
+    """Input rows of `model` for every (genome, tumour) pair, genome-major:
+    the model's own `features` method, else the module-level one."""
+    T = len(descriptors)
+    pi, ti = np.repeat(np.arange(len(g)), T), np.tile(np.arange(T), len(g))
+    f = getattr(model, "features", None)
+    return f(g, descriptors, pi, ti) if f is not None else S.features(g, descriptors, pi, ti)
+
+
 def surrogate_fitness(model, genomes, descriptors):
     """Predicted fitness: 1500 minus the predicted mean count over tumours."""
     g = np.asarray(genomes, np.float32)
-    T = len(descriptors)
-    X = S.features(g, descriptors, np.repeat(np.arange(len(g)), T), np.tile(np.arange(T), len(g)))
-    return FITNESS_OFFSET - model.predict(X).reshape(len(g), T).mean(axis=1)
+    X = _rows(model, g, descriptors)
+    return FITNESS_OFFSET - model.predict(X).reshape(len(g), len(descriptors)).mean(axis=1)
 
 
 def run_surrogate_ga(rng, fitness, model, descriptors, config, pop_size=100, generations=50,
@@ -153,7 +161,6 @@ def run_surrogate_ga(rng, fitness, model, descriptors, config, pop_size=100, gen
     evals = len(population)
     _record(history, t0, 0, fit, population, evals)
     seen_x, seen_y = [], []
-    T = len(descriptors)
     err = []
     gen = 0
     while gen < generations:
@@ -190,9 +197,7 @@ def run_surrogate_ga(rng, fitness, model, descriptors, config, pop_size=100, gen
                 f"{history[-1]['best_so_far']:.2f} screen MAE {err[-1]:.2f} "
                 f"({history[-1]['seconds']:.1f} s)")
         if refit_steps and counts is not None and hasattr(model, "refit"):
-            g = np.asarray(chosen, np.float32)
-            seen_x.append(S.features(g, descriptors, np.repeat(np.arange(len(g)), T),
-                                     np.tile(np.arange(T), len(g))))
+            seen_x.append(_rows(model, np.asarray(chosen, np.float32), descriptors))
             seen_y.append(counts.reshape(-1))
             model.refit(np.concatenate(seen_x), np.concatenate(seen_y), refit_steps)
     return history

@@ -2,7 +2,7 @@
 comparison, as a command line tool.
 
   python -m emt6ro.ga.experiment data --out data.npz [--protocols 1200]
-  python -m emt6ro.ga.experiment compare --data data.npz --out result.json
+  python -m emt6ro.ga.experiment compare --data data.npz [more.npz ...] --out result.json
 
 `data` simulates random dose protocols on synthetic 53x53 tumours (disk radii
 8-14) and records, per (protocol, tumour), the surrogate features and the mean
@@ -76,14 +76,16 @@ def simulate_means(params, tumors, genomes, reps, seed, epoch=0):
 def cmd_data(a):
     params = sim.load_parameters(a.params)
     ts, desc = tumours(params, a.tumours, a.seed)
-    genomes = make_protocols(a.protocols, a.seed + 100)
+    pseed = a.seed + 100 if a.protocol_seed is None else a.protocol_seed
+    genomes = make_protocols(a.protocols, pseed)
     P, T = len(genomes), len(ts)
     mean = np.zeros((P, T))
     sd = np.zeros((P, T))
     t0 = time.perf_counter()
     for p0 in range(0, P, a.chunk):
         p1 = min(P, p0 + a.chunk)
-        m, s, ms = simulate_means(params, ts, genomes[p0:p1], a.reps, a.seed, epoch=p0)
+        m, s, ms = simulate_means(params, ts, genomes[p0:p1], a.reps, a.seed,
+                                  epoch=a.epoch_base + p0)
         mean[p0:p1], sd[p0:p1] = m, s
         th = (p1 - p0) * T * a.reps * HOURS
         log(f"[data] protocols {p0}-{p1}: {ms / 1e3:.1f} s GPU, {th / (ms / 1e3):.0f} "
@@ -95,22 +97,31 @@ def cmd_data(a):
         f"{time.perf_counter() - t0:.1f} s -> {a.out}")
 
 
-def load_samples(path):
-    z = np.load(path)
-    g, desc, mean = z["genomes"], z["descriptors"], z["mean"]
+def load_samples(paths, params=None):
+    """Features, targets (mean final count) and protocol index of the
+    samples in one or more `data` files of the same tumours."""
+    if isinstance(paths, (str, os.PathLike)):
+        paths = [paths]
+    zs = [np.load(p) for p in paths]
+    desc = zs[0]["descriptors"]
+    for z in zs[1:]:
+        if not np.array_equal(z["descriptors"], desc):
+            raise ValueError("data files hold different tumours")
+    g = np.concatenate([z["genomes"] for z in zs])
+    mean = np.concatenate([z["mean"] for z in zs])
     P, T = mean.shape
     pi = np.repeat(np.arange(P), T)
     ti = np.tile(np.arange(T), P)
-    return S.features(g, desc, pi, ti), mean.reshape(-1), pi, z
+    return S.features(g, desc, pi, ti, params), mean.reshape(-1), pi, zs[0]
 
 
-def train_surrogate(X, y, pi, a):
+def train_surrogate(X, y, pi, a, params=None):
     """Split by protocol (10% of protocols held out), fit, report MAE."""
     rng = np.random.default_rng(a.seed)
     P = pi.max() + 1
     held = rng.random(P) < 0.1
     te = held[pi]
-    model = S.Surrogate(seed=a.seed)
+    model = S.Surrogate(seed=a.seed, members=a.members, params=params)
     t0 = time.perf_counter()
     losses = model.fit(X[~te], y[~te], steps=a.train_steps, batch=a.batch, lr=a.lr,
                        seed=a.seed)
@@ -118,7 +129,8 @@ def train_surrogate(X, y, pi, a):
     pred = model.predict(X[te])
     mae = float(np.abs(pred - y[te]).mean())
     out = {"train_samples": int((~te).sum()), "test_samples": int(te.sum()),
-           "train_steps": a.train_steps, "batch": a.batch, "train_seconds": fit_s,
+           "train_steps": a.train_steps, "batch": a.batch, "members": a.members,
+           "train_seconds": fit_s,
            "final_loss": float(losses[-50:].mean()), "mae": mae,
            "test_mean": float(y[te].mean()), "mae_fraction_of_mean": mae / float(y[te].mean()),
            "predict_the_mean_mae": float(np.abs(y[te] - y[~te].mean()).mean())}
@@ -128,8 +140,8 @@ def train_surrogate(X, y, pi, a):
 
 def cmd_compare(a):
     params = sim.load_parameters(a.params)
-    X, y, pi, z = load_samples(a.data)
-    model, sur = train_surrogate(X, y, pi, a)
+    X, y, pi, z = load_samples(a.data, params)
+    model, sur = train_surrogate(X, y, pi, a, params)
     ts, desc = tumours(params, 10, a.seed)
     ga_t, ga_d = ts[:3], desc[:3]
     cfg = ops.default_config()
@@ -195,18 +207,23 @@ def main(argv=None):
     d.add_argument("--tumours", type=int, default=10)
     d.add_argument("--reps", type=int, default=10)
     d.add_argument("--chunk", type=int, default=200)
-    d.add_argument("--seed", type=int, default=0)
+    d.add_argument("--seed", type=int, default=0, help="tumours (and replicate streams)")
+    d.add_argument("--protocol-seed", type=int, default=None,
+                   help="protocols (default: seed + 100)")
+    d.add_argument("--epoch-base", type=int, default=0,
+                   help="replicate-stream epoch of the first chunk (use a different one per file)")
     d.add_argument("--params", default=DEFAULT_PARAMS)
     c = sub.add_parser("compare")
-    c.add_argument("--data", required=True)
+    c.add_argument("--data", required=True, nargs="+", help="one or more data files")
     c.add_argument("--out", required=True)
     c.add_argument("--pop", type=int, default=100)
     c.add_argument("--generations", type=int, default=50)
     c.add_argument("--reps", type=int, default=20)
     c.add_argument("--pool", type=int, default=10000)
     c.add_argument("--verify", type=int, default=200)
-    c.add_argument("--train-steps", type=int, default=3000)
-    c.add_argument("--batch", type=int, default=4096)
+    c.add_argument("--train-steps", type=int, default=300)
+    c.add_argument("--batch", type=int, default=2048)
+    c.add_argument("--members", type=int, default=5, help="networks in the ensemble")
     c.add_argument("--lr", type=float, default=1e-3)
     c.add_argument("--refit-steps", type=int, default=0)
     c.add_argument("--skip-plain", action="store_true")

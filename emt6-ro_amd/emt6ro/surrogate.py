@@ -1,4 +1,4 @@
-"""Learned fitness surrogate: an MLP from (dose-time vector, tumour descriptor)
+"""Learned fitness surrogate: an MLP from (dose protocol, tumour descriptor)
 to the mean final living-cell count, trained and evaluated on the GPU by the
 library's BF16 matrix-core kernels (include/emt6ro_amd.h, emt6ro_mlp_*).
 
@@ -8,10 +8,24 @@ loads an external pretrained forecasting model (python_ga/example_run_model.py:
 GA screens candidates with the surrogate and verifies the best of them by
 simulation (emt6ro.ga).
 
-Input features (152 by default): doses in Gy on 144 half-hour slots (the
-first 72 h), then 8 descriptors of the initial tumour: living cells, radius,
-fractions of living cells in G1, S, G2 and M (D counted with M), and mean
-oxygen and glucose over the living cells.
+Input features (36): 8 descriptors of the initial tumour (living cells,
+radius, fractions of living cells in G1, S, G2 and M (D counted with M),
+mean oxygen and glucose over the living cells), then 28 features of the
+dose protocol (doses in Gy on 144 half-hour slots, the first 72 h):
+
+- total dose, number of doses, dose-weighted mean time, first and last dose
+  time, and the dose in each 12 h window (11 features);
+- a simplified radiation-damage history of one cell under the model's
+  repair rules (cell.cu:84-100, cell.h:104-133), with the repair timer
+  started at each dose rather than at the next cycle change
+  (`protocol_features`): log-survival over all repair completions, kill
+  probability weighted by the time left to 144 h, number of repair
+  completions, largest accumulated dose, total time in repair, and the
+  log-survival at every 6 h up to 72 h (17 features).
+
+Raw per-slot doses are not inputs: with about a thousand training
+protocols they let the network fit noise (held-out error 27.6 vs 18.5
+counts in a CPU comparison on the same data).
 """
 import ctypes as C
 
@@ -21,7 +35,16 @@ from ._native import lib, check_mlp
 
 N_DOSE_SLOTS = 144
 N_DESCRIPTORS = 8
-N_FEATURES = N_DOSE_SLOTS + N_DESCRIPTORS
+N_PROTOCOL_FEATURES = 28
+N_FEATURES = N_DESCRIPTORS + N_PROTOCOL_FEATURES
+SLOT_HOURS = 0.5
+HORIZON_HOURS = 144.0
+
+# Repair constants in the flat parameter vector of include/emt6ro_amd.h
+# (delay coefficient and exponent, survival coefficient and exponent, repair
+# half time), and their values in data/default-parameters.json.
+_REPAIR_INDEX = (35, 36, 37, 38, 39)
+DEFAULT_REPAIR = (3.3414, 0.1492, 1.0, -0.4993, 0.8)
 
 
 def _fp(a):
@@ -48,15 +71,99 @@ def tumour_descriptor(state):
     return out
 
 
-def features(dose_rows, descriptors, index_dose, index_tumour):
-    """Rows [dose_rows[index_dose[i]] (first 144 slots), descriptors[index_tumour[i]]]."""
-    d = np.asarray(dose_rows, np.float32)
-    if d.shape[1] < N_DOSE_SLOTS:
-        d = np.pad(d, ((0, 0), (0, N_DOSE_SLOTS - d.shape[1])))
-    d = d[:, :N_DOSE_SLOTS]
+def repair_constants(params=None):
+    """(delay coeff, delay exp, survival coeff, survival exp, repair half
+    time) of a Parameters object or flat parameter vector; the defaults
+    without one."""
+    if params is None:
+        return DEFAULT_REPAIR
+    v = params.values() if hasattr(params, "values") and callable(params.values) else params
+    v = np.asarray(v, np.float64)
+    return tuple(float(v[i]) for i in _REPAIR_INDEX)
+
+
+def protocol_features(dose_rows, params=None):
+    """(P, 28) features of P dose protocols (see the module docstring).
+
+    The damage history follows the model's rules in continuous time: a dose
+    d at time t sets R = R / (1 + (t - t0) / half_time) + d if the cell is
+    still repairing its previous damage (t0: time of the previous dose), or
+    R = d; repair ends at t0 + coeff * exp(exp_coeff * R), when the cell
+    dies with probability 1 - surv_coeff * exp(surv_exp * R) or else starts
+    clean (R = 0)."""
+    dc, de, sc, se, half = repair_constants(params)
+    G = np.asarray(dose_rows, np.float64)
+    if G.ndim == 1:
+        G = G[None]
+    if G.shape[1] < N_DOSE_SLOTS:
+        G = np.pad(G, ((0, 0), (0, N_DOSE_SLOTS - G.shape[1])))
+    G = G[:, :N_DOSE_SLOTS]
+    P = G.shape[0]
+    t = np.arange(N_DOSE_SLOTS) * SLOT_HOURS
+    given = G > 0
+    tot = G.sum(1)
+    out = np.zeros((P, N_PROTOCOL_FEATURES))
+    out[:, 0] = tot
+    out[:, 1] = given.sum(1)
+    out[:, 2] = (G * t).sum(1) / np.maximum(tot, 1e-12)
+    any_dose = given.any(1)
+    out[:, 3] = np.where(any_dose, t[np.argmax(given, 1)], 0.0)
+    out[:, 4] = np.where(any_dose, t[N_DOSE_SLOTS - 1 - np.argmax(given[:, ::-1], 1)], 0.0)
+    out[:, 5:11] = G.reshape(P, 6, 24).sum(2)
+    R = np.zeros(P)
+    t0 = np.zeros(P)          # last dose
+    ep0 = np.zeros(P)         # first dose of the current repair episode
+    rep = np.zeros(P, bool)
+    log_s = np.zeros(P)
+    kill_w = np.zeros(P)
+    n_rep = np.zeros(P)
+    max_r = np.zeros(P)
+    t_rep = np.zeros(P)
+
+    def complete(before):
+        nonlocal R, rep, log_s, kill_w, n_rep, t_rep
+        end = t0 + dc * np.exp(de * R)
+        done = rep & (end <= before)
+        if done.any():
+            surv = np.clip(sc * np.exp(se * R), 1e-12, 1.0)
+            log_s = np.where(done, log_s + np.log(surv), log_s)
+            kill_w = np.where(done, kill_w + (1.0 - surv) * np.maximum(HORIZON_HOURS - end, 0.0),
+                              kill_w)
+            n_rep = n_rep + done
+            t_rep = np.where(done, t_rep + (end - ep0), t_rep)
+            R = np.where(done, 0.0, R)
+            rep = rep & ~done
+
+    k = 16
+    for s in range(N_DOSE_SLOTS):
+        ts = t[s]
+        complete(ts)
+        d = G[:, s]
+        hit = d > 0
+        R = np.where(hit, np.where(rep, R / (1.0 + (ts - t0) / half), 0.0) + d, R)
+        ep0 = np.where(hit & ~rep, ts, ep0)
+        t0 = np.where(hit, ts, t0)
+        rep = rep | hit
+        max_r = np.maximum(max_r, R)
+        if (s + 1) % 12 == 0:
+            complete(ts + SLOT_HOURS)
+            out[:, k] = log_s
+            k += 1
+    complete(np.inf)
+    out[:, 11] = log_s
+    out[:, 12] = kill_w
+    out[:, 13] = n_rep
+    out[:, 14] = max_r
+    out[:, 15] = t_rep
+    return out
+
+
+def features(dose_rows, descriptors, index_dose, index_tumour, params=None):
+    """Rows [descriptors[index_tumour[i]], protocol_features(dose_rows)[index_dose[i]]]."""
+    f = protocol_features(dose_rows, params).astype(np.float32)
     t = np.asarray(descriptors, np.float32).reshape(-1, N_DESCRIPTORS)
     return np.ascontiguousarray(
-        np.concatenate([d[np.asarray(index_dose)], t[np.asarray(index_tumour)]], axis=1))
+        np.concatenate([t[np.asarray(index_tumour)], f[np.asarray(index_dose)]], axis=1))
 
 
 class MLP:
@@ -136,19 +243,42 @@ class MLP:
 
 
 class Surrogate:
-    """The MLP with per-feature input standardisation and a standardised
-    target, fitted on (features, mean final living count) pairs."""
+    """An ensemble of `members` MLPs (different initialisations and
+    minibatch streams) with per-feature input standardisation and a
+    standardised target, fitted on (features, mean final living count)
+    pairs; predictions are the members' mean."""
 
-    def __init__(self, in_features=N_FEATURES, hidden=512, seed=0, gpu_id=0):
-        self.net = MLP(in_features, hidden, seed, gpu_id)
+    def __init__(self, in_features=N_FEATURES, hidden=512, seed=0, gpu_id=0, members=1,
+                 params=None):
+        if members < 1:
+            raise ValueError("members must be >= 1")
+        self.params = params   # repair constants for features(); None: the defaults
+        self.nets = [MLP(in_features, hidden, seed + 7919 * m, gpu_id) for m in range(members)]
         self.x_mu = np.zeros(in_features, np.float32)
         self.x_sd = np.ones(in_features, np.float32)
         self.y_mu, self.y_sd = 0.0, 1.0
 
+    @property
+    def net(self):
+        return self.nets[0]
+
+    def features(self, dose_rows, descriptors, index_dose, index_tumour):
+        """The input rows of this surrogate (module-level `features` with its
+        parameters)."""
+        return features(dose_rows, descriptors, index_dose, index_tumour, self.params)
+
     def _nx(self, X):
         return (np.asarray(X, np.float32) - self.x_mu) / self.x_sd
 
-    def fit(self, X, y, steps=2000, batch=4096, lr=1e-3, seed=0):
+    def _train_all(self, Xn, yn, steps, batch, lr, seed):
+        losses = []
+        for m, net in enumerate(self.nets):
+            net.set_data(Xn, yn)
+            losses.append(net.train(steps, batch=batch, lr=lr, seed=seed + 104729 * m))
+        return np.mean(losses, axis=0)
+
+    def fit(self, X, y, steps=300, batch=2048, lr=1e-3, seed=0):
+        """Returns the members' mean training loss per step."""
         X = np.asarray(X, np.float32)
         y = np.asarray(y, np.float64)
         self.x_mu = X.mean(axis=0).astype(np.float32)
@@ -156,8 +286,8 @@ class Surrogate:
         self.x_sd = np.where(sd > 1e-6, sd, 1.0).astype(np.float32)
         self.y_mu, self.y_sd = float(y.mean()), float(max(y.std(), 1e-6))
         self._base = (X, y)
-        self.net.set_data(self._nx(X), ((y - self.y_mu) / self.y_sd).astype(np.float32))
-        return self.net.train(steps, batch=min(batch, len(y)), lr=lr, seed=seed)
+        return self._train_all(self._nx(X), ((y - self.y_mu) / self.y_sd).astype(np.float32),
+                               steps, min(batch, len(y)), lr, seed)
 
     def refit(self, X, y, steps, batch=1024, lr=3e-4, seed=1):
         """Fine-tune on new samples mixed with the original training set (the
@@ -169,9 +299,11 @@ class Surrogate:
         rep = max(1, int(np.ceil(len(by) / (3.0 * len(y)))))
         Xa = np.concatenate([bx, np.tile(X, (rep, 1))])
         ya = np.concatenate([by, np.tile(y, rep)])
-        self.net.set_data(self._nx(Xa), ((ya - self.y_mu) / self.y_sd).astype(np.float32))
         self._refits = getattr(self, "_refits", 0) + 1
-        return self.net.train(steps, batch=min(batch, len(ya)), lr=lr, seed=seed + self._refits)
+        return self._train_all(self._nx(Xa), ((ya - self.y_mu) / self.y_sd).astype(np.float32),
+                               steps, min(batch, len(ya)), lr, seed + self._refits)
 
     def predict(self, X):
-        return self.net.predict(self._nx(X)).astype(np.float64) * self.y_sd + self.y_mu
+        xn = self._nx(X)
+        out = np.mean([net.predict(xn).astype(np.float64) for net in self.nets], axis=0)
+        return out * self.y_sd + self.y_mu

@@ -191,7 +191,11 @@ class FakeSim:
 
 
 class FakeModel:
-    """A perfect surrogate: reads the doses back out of the feature rows."""
+    """A perfect surrogate: its rows are the doses and the tumour id."""
+
+    def features(self, g, desc, pi, ti):
+        g = np.asarray(g, np.float32)
+        return np.concatenate([g[pi], np.asarray(desc, np.float32)[ti]], axis=1)
 
     def predict(self, X):
         X = np.asarray(X)

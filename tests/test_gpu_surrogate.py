@@ -150,11 +150,18 @@ def test_surrogate_standardises_and_errors(S):
     log("[surrogate] wrapper")
     rng = np.random.default_rng(0)
     X = rng.random((3000, S.N_FEATURES)).astype(np.float32) * 2.0
-    y = 500.0 + 100.0 * X[:, 0] - 50.0 * X[:, 150]
-    s = S.Surrogate(seed=1)
+    y = 500.0 + 100.0 * X[:, 0] - 50.0 * X[:, S.N_FEATURES - 1]
+    s = S.Surrogate(seed=1, members=2)
     losses = s.fit(X, y, steps=400, batch=512, lr=1e-3, seed=2)
     assert losses[-10:].mean() < losses[:10].mean()
     pred = s.predict(X[:500])
     assert np.abs(pred - y[:500]).mean() < 0.1 * np.abs(y[:500] - y.mean()).mean() + 5.0
+    # The ensemble predicts its members' mean.
+    xn = s._nx(X[:50])
+    members = [net.predict(xn).astype(np.float64) * s.y_sd + s.y_mu for net in s.nets]
+    assert np.allclose(s.predict(X[:50]), np.mean(members, axis=0), rtol=1e-6, atol=1e-6)
+    assert not np.array_equal(members[0], members[1])
     with pytest.raises(ValueError):
         s.net.predict(np.zeros((2, 3), np.float32))
+    with pytest.raises(ValueError):
+        S.Surrogate(members=0)
