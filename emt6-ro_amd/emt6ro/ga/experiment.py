@@ -221,7 +221,7 @@ def main(argv=None):
     c.add_argument("--reps", type=int, default=20)
     c.add_argument("--pool", type=int, default=10000)
     c.add_argument("--verify", type=int, default=200)
-    c.add_argument("--train-steps", type=int, default=300)
+    c.add_argument("--train-steps", type=int, default=500)
     c.add_argument("--batch", type=int, default=2048)
     c.add_argument("--members", type=int, default=5, help="networks in the ensemble")
     c.add_argument("--lr", type=float, default=1e-3)

@@ -8,24 +8,26 @@ loads an external pretrained forecasting model (python_ga/example_run_model.py:
 GA screens candidates with the surrogate and verifies the best of them by
 simulation (emt6ro.ga).
 
-Input features (36): 8 descriptors of the initial tumour (living cells,
+Input features (71): 8 descriptors of the initial tumour (living cells,
 radius, fractions of living cells in G1, S, G2 and M (D counted with M),
-mean oxygen and glucose over the living cells), then 28 features of the
+mean oxygen and glucose over the living cells), then 63 features of the
 dose protocol (doses in Gy on 144 half-hour slots, the first 72 h):
 
 - total dose, number of doses, dose-weighted mean time, first and last dose
   time, and the dose in each 12 h window (11 features);
-- a simplified radiation-damage history of one cell under the model's
-  repair rules (cell.cu:84-100, cell.h:104-133), with the repair timer
-  started at each dose rather than at the next cycle change
-  (`protocol_features`): log-survival over all repair completions, kill
+- the radiation-damage history of one cell under the model's repair rules
+  (cell.cu:84-100, cell.h:104-133), in which a dose resets the repair timer
+  and the timer restarts only at the cell's next cycle change: with the
+  timer restarting at once, log-survival over all repair completions, kill
   probability weighted by the time left to 144 h, number of repair
   completions, largest accumulated dose, total time in repair, and the
-  log-survival at every 6 h up to 72 h (17 features).
+  log-survival at every 6 h up to 72 h (17 features); then the first five
+  of these for restart waits of 1, 2, 3, 4, 6, 8 and 12 h (35 features).
 
 Raw per-slot doses are not inputs: with about a thousand training
 protocols they let the network fit noise (held-out error 27.6 vs 18.5
-counts in a CPU comparison on the same data).
+counts in a CPU comparison on the same data), and the waits matter (16.4
+vs 11.3 counts with 3,600 protocols).
 """
 import ctypes as C
 
@@ -35,7 +37,10 @@ from ._native import lib, check_mlp
 
 N_DOSE_SLOTS = 144
 N_DESCRIPTORS = 8
-N_PROTOCOL_FEATURES = 28
+# Waits (h) from a dose to the start of the repair timer in the extra damage
+# histories.
+REPAIR_WAITS = (1.0, 2.0, 3.0, 4.0, 6.0, 8.0, 12.0)
+N_PROTOCOL_FEATURES = 11 + 17 + 5 * len(REPAIR_WAITS)
 N_FEATURES = N_DESCRIPTORS + N_PROTOCOL_FEATURES
 SLOT_HOURS = 0.5
 HORIZON_HOURS = 144.0
@@ -82,36 +87,23 @@ def repair_constants(params=None):
     return tuple(float(v[i]) for i in _REPAIR_INDEX)
 
 
-def protocol_features(dose_rows, params=None):
-    """(P, 28) features of P dose protocols (see the module docstring).
+def _damage_history(G, wait, constants, snapshots):
+    """Damage history of one cell for protocols G (P, 144 slots): columns
+    log-survival, time-weighted kill probability, repair completions,
+    largest accumulated dose, time in repair, then (with `snapshots`) the
+    log-survival at every 6 h up to 72 h.
 
-    The damage history follows the model's rules in continuous time: a dose
-    d at time t sets R = R / (1 + (t - t0) / half_time) + d if the cell is
-    still repairing its previous damage (t0: time of the previous dose), or
-    R = d; repair ends at t0 + coeff * exp(exp_coeff * R), when the cell
-    dies with probability 1 - surv_coeff * exp(surv_exp * R) or else starts
-    clean (R = 0)."""
-    dc, de, sc, se, half = repair_constants(params)
-    G = np.asarray(dose_rows, np.float64)
-    if G.ndim == 1:
-        G = G[None]
-    if G.shape[1] < N_DOSE_SLOTS:
-        G = np.pad(G, ((0, 0), (0, N_DOSE_SLOTS - G.shape[1])))
-    G = G[:, :N_DOSE_SLOTS]
+    A dose d at time t gives R = R / (1 + t_rep / half_time) + d, where
+    t_rep is the time the repair timer has run since the previous dose, and
+    resets the timer, which restarts `wait` hours later (in the model, at
+    the cell's next cycle change); repair ends when the timer reaches
+    coeff * exp(exp_coeff * R), and the cell then dies with probability
+    1 - surv_coeff * exp(surv_exp * R) or starts clean (R = 0)."""
+    dc, de, sc, se, half = constants
     P = G.shape[0]
     t = np.arange(N_DOSE_SLOTS) * SLOT_HOURS
-    given = G > 0
-    tot = G.sum(1)
-    out = np.zeros((P, N_PROTOCOL_FEATURES))
-    out[:, 0] = tot
-    out[:, 1] = given.sum(1)
-    out[:, 2] = (G * t).sum(1) / np.maximum(tot, 1e-12)
-    any_dose = given.any(1)
-    out[:, 3] = np.where(any_dose, t[np.argmax(given, 1)], 0.0)
-    out[:, 4] = np.where(any_dose, t[N_DOSE_SLOTS - 1 - np.argmax(given[:, ::-1], 1)], 0.0)
-    out[:, 5:11] = G.reshape(P, 6, 24).sum(2)
     R = np.zeros(P)
-    t0 = np.zeros(P)          # last dose
+    t_last = np.zeros(P)      # last dose
     ep0 = np.zeros(P)         # first dose of the current repair episode
     rep = np.zeros(P, bool)
     log_s = np.zeros(P)
@@ -119,10 +111,11 @@ def protocol_features(dose_rows, params=None):
     n_rep = np.zeros(P)
     max_r = np.zeros(P)
     t_rep = np.zeros(P)
+    snaps = []
 
     def complete(before):
         nonlocal R, rep, log_s, kill_w, n_rep, t_rep
-        end = t0 + dc * np.exp(de * R)
+        end = t_last + wait + dc * np.exp(de * R)
         done = rep & (end <= before)
         if done.any():
             surv = np.clip(sc * np.exp(se * R), 1e-12, 1.0)
@@ -134,28 +127,49 @@ def protocol_features(dose_rows, params=None):
             R = np.where(done, 0.0, R)
             rep = rep & ~done
 
-    k = 16
     for s in range(N_DOSE_SLOTS):
         ts = t[s]
         complete(ts)
         d = G[:, s]
         hit = d > 0
-        R = np.where(hit, np.where(rep, R / (1.0 + (ts - t0) / half), 0.0) + d, R)
+        ran = np.maximum(ts - (t_last + wait), 0.0)
+        R = np.where(hit, np.where(rep, R / (1.0 + ran / half), 0.0) + d, R)
         ep0 = np.where(hit & ~rep, ts, ep0)
-        t0 = np.where(hit, ts, t0)
+        t_last = np.where(hit, ts, t_last)
         rep = rep | hit
         max_r = np.maximum(max_r, R)
-        if (s + 1) % 12 == 0:
+        if snapshots and (s + 1) % 12 == 0:
             complete(ts + SLOT_HOURS)
-            out[:, k] = log_s
-            k += 1
+            snaps.append(log_s.copy())
     complete(np.inf)
-    out[:, 11] = log_s
-    out[:, 12] = kill_w
-    out[:, 13] = n_rep
-    out[:, 14] = max_r
-    out[:, 15] = t_rep
-    return out
+    return np.stack([log_s, kill_w, n_rep, max_r, t_rep] + snaps, axis=1)
+
+
+def protocol_features(dose_rows, params=None):
+    """(P, N_PROTOCOL_FEATURES) features of P dose protocols (see the
+    module docstring)."""
+    constants = repair_constants(params)
+    G = np.asarray(dose_rows, np.float64)
+    if G.ndim == 1:
+        G = G[None]
+    if G.shape[1] < N_DOSE_SLOTS:
+        G = np.pad(G, ((0, 0), (0, N_DOSE_SLOTS - G.shape[1])))
+    G = G[:, :N_DOSE_SLOTS]
+    P = G.shape[0]
+    t = np.arange(N_DOSE_SLOTS) * SLOT_HOURS
+    given = G > 0
+    tot = G.sum(1)
+    stats = np.zeros((P, 11))
+    stats[:, 0] = tot
+    stats[:, 1] = given.sum(1)
+    stats[:, 2] = (G * t).sum(1) / np.maximum(tot, 1e-12)
+    any_dose = given.any(1)
+    stats[:, 3] = np.where(any_dose, t[np.argmax(given, 1)], 0.0)
+    stats[:, 4] = np.where(any_dose, t[N_DOSE_SLOTS - 1 - np.argmax(given[:, ::-1], 1)], 0.0)
+    stats[:, 5:11] = G.reshape(P, 6, 24).sum(2)
+    parts = [stats, _damage_history(G, 0.0, constants, True)]
+    parts += [_damage_history(G, w, constants, False) for w in REPAIR_WAITS]
+    return np.concatenate(parts, axis=1)
 
 
 def features(dose_rows, descriptors, index_dose, index_tumour, params=None):

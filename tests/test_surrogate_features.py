@@ -95,3 +95,30 @@ def test_rows_and_row_mapping():
     longer = np.pad(g, ((0, 0), (0, 10)))
     longer[:, -1] = 3.0
     assert np.array_equal(S.protocol_features(longer), S.protocol_features(g))
+
+
+def waits_block(f, w):
+    """The five summary columns of the history with restart wait w."""
+    k = 28 + 5 * S.REPAIR_WAITS.index(w)
+    return f[k:k + 5]
+
+
+def test_waits_delay_the_repair_and_keep_doses_undecayed():
+    assert S.N_PROTOCOL_FEATURES == 28 + 5 * len(S.REPAIR_WAITS)
+    f = S.protocol_features(protocol({2.0: 2.0}))[0]
+    log_s, kill_w, n_rep, max_r, t_rep = waits_block(f, 4.0)
+    end = 2.0 + 4.0 + delay(2.0)
+    assert log_s == pytest.approx(SE * 2.0) and n_rep == 1 and max_r == 2.0
+    assert kill_w == pytest.approx((1 - math.exp(SE * 2.0)) * (144.0 - end))
+    assert t_rep == pytest.approx(4.0 + delay(2.0))
+    # A second dose while the timer waits adds in full: R = 1 + 1.
+    f = S.protocol_features(protocol({0.0: 1.0, 1.0: 1.0}))[0]
+    for w in (1.0, 2.0):
+        log_s, kill_w, n_rep, max_r, t_rep = waits_block(f, w)
+        assert max_r == 2.0 and n_rep == 1
+        assert t_rep == pytest.approx(1.0 + w + delay(2.0))
+    # Once the timer runs, the earlier damage decays: 1 h wait, dose at 2 h.
+    f = S.protocol_features(protocol({0.0: 1.0, 2.0: 1.0}))[0]
+    R = 1.0 / (1.0 + 1.0 / HALF) + 1.0
+    assert waits_block(f, 1.0)[3] == pytest.approx(R)
+    assert waits_block(f, 2.0)[3] == 2.0
