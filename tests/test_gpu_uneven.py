@@ -55,6 +55,39 @@ def test_uneven_sizes_and_horizons_match_oracle():
         compare_states(b.state(i), orc, 0, f"uneven sim {i} ({sizes[t]}x{sizes[t]})")
 
 
+def test_large_rois_match_oracle():
+    """ROIs large enough that the lane-run diffuser's kept list of masked
+    ROI sites overflows on some threads (its per-step loop takes over),
+    next to ones that fit, on 53x53 and 81x81 lattices."""
+    import emt6ro.simulation as sim
+    from emt6ro.batch import Batch
+    from emt6ro.synthetic import disk_state
+    params = sim.load_parameters(golden("default-parameters.json"))
+    v = params.values()
+    sizes = [53, 53, 81, 81]
+    radii = [20, 24, 28, 36]
+    tumors, orc_states = [], []
+    for k, (sz, r) in enumerate(zip(sizes, radii)):
+        st, arr = disk_state(params, sz, sz, radius=r, seed=70 + k)
+        tumors.append(st)
+        orc_states.append(oracle_state_from_arrays(arr, v))
+    n = len(tumors)
+    doses = np.zeros((1, 480), np.float32)
+    doses[0, 0] = 1.5
+    steps = 300
+    b = Batch(params, tumors, np.arange(n), np.zeros(n, np.int64),
+              np.full(n, steps, np.uint32), doses, 300, 144000, sim_id=np.arange(n), seed=5)
+    log(f"[uneven] large ROIs: radii {radii}, {steps} steps")
+    b.run(steps)
+    got = b.results().astype(np.int64)
+    for i in range(n):
+        orc = O.OracleBatch(v, [orc_states[i]], doses, [0], [i], 300, 144000)
+        orc.run(steps, seed=5, nthreads=4)
+        assert got[i] == orc.counts()[0], (i, got[i], orc.counts()[0])
+        compare_states(b.state(i), orc, 0, f"large ROI sim {i} ({sizes[i]}x{sizes[i]}, r {radii[i]})")
+        log(f"  sim {i}: ROI {orc.roi[0].tolist()} living {got[i]}")
+
+
 def test_autotuner_keeps_results():
     import emt6ro.simulation as sim
     from emt6ro.batch import Batch
