@@ -148,12 +148,12 @@ def cmd_compare(a):
     result = {"surrogate": sur, "config": {
         "pop_size": a.pop, "generations": a.generations, "reps": a.reps, "pool": a.pool,
         "verify": a.verify, "tumours": 3, "budget_gy": cfg["max_value"], "hours": HOURS}}
-    judge = D.SimulatorFitness(params, ga_t, a.reeval_reps, hours=HOURS, seed=a.seed + 99)
+    reeval = D.SimulatorFitness(params, ga_t, a.reeval_reps, hours=HOURS, seed=a.seed + 99)
 
     def reevaluate(hist):
         """The best genome found, re-simulated with fresh replicates."""
         best = max(hist, key=lambda h: h["best"])
-        return float(judge([np.asarray(best["best_genome"], np.float32)])[0])
+        return float(reeval([np.asarray(best["best_genome"], np.float32)])[0])
 
     def save():
         with open(a.out, "w") as f:

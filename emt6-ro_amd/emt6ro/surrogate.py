@@ -4,7 +4,7 @@ library's BF16 matrix-core kernels (include/emt6ro_amd.h, emt6ro_mlp_*).
 
 The reference has no surrogate of its own that runs offline: its GA example
 loads an external pretrained forecasting model (python_ga/example_run_model.py:
-24-184).  Here the simulator makes the training data and stays the judge: the
+24-184).  Here the simulator makes the training data and has the last word: the
 GA screens candidates with the surrogate and verifies the best of them by
 simulation (emt6ro.ga).
 

@@ -20,8 +20,10 @@
  *   - eo_diffuse_f64        FP64 restatement of diffusion/diffusion.cu:119-191
  *                           (the reference formula order, in double); pinned by
  *                           fixed-point, symmetry and brute-force checks.
- *   - eo_diffuse_f32 and    the FP32 simulator with the deterministic rules
- *     eo_run                 written down in DESIGN.md ("step semantics"); the
+ *   - eo_diffuse_f32 and    the FP32 simulator with the build's deterministic
+ *     eo_run                 step rules (division arbitration by key, child
+ *                           vacancy after all deaths, seeded counter-based
+ *                           draws; stated beside each function below); the
  *                           FP32 diffusion is pinned to eo_diffuse_f64 within
  *                           1e-5, the bookkeeping by the tests listed above.
  */
