@@ -199,3 +199,32 @@ def test_reset_restores_initial_state(sim, params):
     b.reset(epoch=0)
     b.run(700)
     assert np.array_equal(b.results(), first)
+
+
+@pytest.mark.parametrize("path", ["persistent", "step"])
+def test_empty_and_tiny_lattices_match_oracle(sim, params, path):
+    """A lattice without cells (no ROI: diffusion skipped, Q8) and a
+    two-site-radius colony, next to each other in one batch."""
+    from emt6ro.batch import Batch
+    from emt6ro.synthetic import disk_state
+    v = params.values()
+    empty = sim.TumorState.empty(params, 53, 53)
+    tiny, arr = disk_state(params, 53, 53, radius=2, seed=9)
+    doses = np.zeros((1, 480), np.float32)
+    doses[0, 1] = 1.0
+    n, steps, seed = 4, 600, 77
+    tumor_index = np.array([0, 1, 0, 1])
+    b = Batch(params, [empty, tiny], tumor_index, np.zeros(n, np.int64),
+              np.full(n, 2 ** 32 - 1, np.uint32), doses, 300, 144000, sim_id=np.arange(n),
+              seed=seed)
+    log(f"[parity] empty and tiny lattices, path={path}")
+    b.run(steps, path=path)
+    states = [O.empty_state(v, 53, 53), oracle_state_from_arrays(arr, v)]
+    orc = O.OracleBatch(v, [states[t] for t in tumor_index], doses, np.zeros(n, np.int64),
+                        np.arange(n), 300, 144000)
+    orc.run(steps, seed=seed, nthreads=4)
+    got = b.results().astype(np.int64)
+    assert np.array_equal(got, orc.counts()), (got, orc.counts())
+    assert got[0] == 0 and got[2] == 0 and got[1] > 0
+    for i in range(n):
+        compare_states(b.state(i), orc, i, f"{path} sim {i}")
