@@ -125,11 +125,17 @@ def random_population(rng, n, n_slots, config):
 # ---------------------------------------------------------------------------
 # Selection (selection.py)
 # ---------------------------------------------------------------------------
-def round_select_n(select_n, pop_size):
-    n = round(pop_size * select_n)
-    while n % 2:
-        n -= 1
-    return n
+def even_count(fraction, pop_size):
+    """round(pop_size * fraction) made even by rounding down (selection.py's
+    selection-size rule)."""
+    n = round(pop_size * fraction)
+    return n - (n % 2)
+
+
+def _tournament_size(pop_size, select_n):
+    """Candidates per tournament: the population over the selections,
+    rounded half to even like Python's round()."""
+    return round(pop_size / select_n)
 
 
 def roulette_selection(rng, population, fitness, select_n, config):
@@ -156,7 +162,7 @@ def simple_selection(rng, population, fitness, select_n, config):
 def tournament_selection_classic(rng, population, fitness, select_n, config):
     p = config["probability"]
     fitness = np.asarray(fitness)
-    k = round(len(population) / select_n)
+    k = _tournament_size(len(population), select_n)
     out = []
     for _ in range(select_n):
         cand = [int(rng.integers(0, len(population))) for _ in range(k)]
@@ -170,7 +176,7 @@ def tournament_selection_classic(rng, population, fitness, select_n, config):
 
 def tournament_selection_tuned(rng, population, fitness, select_n, config):
     p = config["probability"]
-    k = round(len(population) / select_n)
+    k = _tournament_size(len(population), select_n)
     out = []
     for i in range(select_n):
         sp = p * (1 - p) ** i
@@ -387,8 +393,8 @@ def next_generation(rng, population, fitness, config, pop_size=None):
     population's size), the retained parents first."""
     n = len(population)
     pop_size = n if pop_size is None else pop_size
-    select_n = round_select_n(config["select_n"] - config["retained_parent_protocols"], n)
-    config["no_of_retaining_parents"] = round_select_n(config["retained_parent_protocols"], n)
+    select_n = even_count(config["select_n"] - config["retained_parent_protocols"], n)
+    config["no_of_retaining_parents"] = even_count(config["retained_parent_protocols"], n)
     keep = simple_selection(rng, population, fitness, config["no_of_retaining_parents"], config)
     chosen = SELECTIONS[config["selection"]["type"]](rng, population, fitness, select_n,
                                                       config["selection"])

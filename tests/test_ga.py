@@ -37,11 +37,13 @@ def test_annealing_schedule_closed_form():
     assert ops.annealing_probability(100, 0.2, 100) == 0.001  # below eps
 
 
-def test_round_select_n_is_even():
+def test_selection_counts_are_even():
     for pop in range(2, 60):
         for frac in (0.1, 0.33, 0.5, 0.9):
-            n = ops.round_select_n(frac, pop)
-            assert n % 2 == 0 and n <= round(pop * frac)
+            n = ops.even_count(frac, pop)
+            assert n % 2 == 0 and round(pop * frac) - 1 <= n <= round(pop * frac)
+    assert ops.even_count(0.5, 5) == 2      # round(2.5) = 2 (half to even)
+    assert ops.even_count(0.7, 5) == 4      # round(3.5) = 4
 
 
 def test_random_protocols_respect_grid_cap_budget_and_spacing():
