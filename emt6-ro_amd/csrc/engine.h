@@ -97,6 +97,7 @@ class Batch {
   uint32_t *slots_ = nullptr;
   float *prot_ = nullptr;
   int32_t *order_ = nullptr;
+  int32_t *order_next_ = nullptr;  // the order of the next launch, sorted by cost
   int32_t *queue_ = nullptr;
   uint32_t *results_ = nullptr;
   int32_t *scratch_ = nullptr;

@@ -173,6 +173,7 @@ Batch::Batch(const HostParams &p, const std::vector<const TumorState *> &tumors,
   slots_ = dalloc<uint32_t>(slot_total_);
   prot_ = dalloc<float>((size_t)spec.n_prot * spec.n_slots);
   order_ = dalloc<int32_t>(nsims_);
+  order_next_ = dalloc<int32_t>(nsims_);
   queue_ = dalloc<int32_t>(1);
   results_ = dalloc<uint32_t>(nsims_);
   EMT_HIP(hipMemcpyAsync(desc_, desc_h_.data(), nsims_ * sizeof(SimDesc), hipMemcpyHostToDevice, stream_));
@@ -208,7 +209,7 @@ Batch::~Batch() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
   dfree(desc_); dfree(hdr_); dfree(sub_); dfree(st_); dfree(slots_); dfree(prot_);
-  dfree(order_); dfree(queue_); dfree(results_); dfree(scratch_); dfree(ovf_);
+  dfree(order_); dfree(order_next_); dfree(queue_); dfree(results_); dfree(scratch_); dfree(ovf_);
   for (auto &p : img_sub_) dfree(p);
   for (auto &p : img_st_) dfree(p);
   for (auto &p : img_slots_) dfree(p);
@@ -393,12 +394,15 @@ void Batch::run(uint32_t nsteps, int path, int variant, uint32_t steps_per_launc
       for (size_t ci = 0; ci < classes_.size(); ++ci) {
         const Class &c = classes_[ci];
         dev::RunArgs a = args(k);
-        a.order = order_ + c.offset;
+        // Longest first by the ROI each simulation reached.
+        EMT_HIP(launch_order_by_cost(order_ + c.offset, order_next_ + c.offset, c.count, hdr_,
+                                     desc_, stream_));
+        a.order = order_next_ + c.offset;
         a.nsims = c.count;
         a.pv_max = vt[plans[ci].v].rs * (c.hmax + 2);
         EMT_HIP(launch_fill_i32(queue_, 0, 1, stream_));
         EMT_HIP(launch_persistent_lds(plans[ci].v, a, plans[ci].grid, plans[ci].lds, stream_));
-        launches += 2;
+        launches += 3;
       }
       done += k;
     }

@@ -41,6 +41,9 @@ hipError_t launch_step(int variant, const dev::RunArgs &a, int nsims, int hmax, 
 hipError_t launch_count_living(const SimDesc *desc, const uint8_t *st_pool, uint32_t *out,
                                int nsims, hipStream_t s);
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int n, hipStream_t s);
+// Longest-first (largest ROI area) copy of the queue order src[0..n) into dst.
+hipError_t launch_order_by_cost(const int32_t *src, int32_t *dst, int n, const SimHdr *hdr,
+                                const SimDesc *desc, hipStream_t s);
 
 }  // namespace emt6ro
 

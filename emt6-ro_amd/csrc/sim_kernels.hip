@@ -302,6 +302,56 @@ __global__ void fill_i32_kernel(int32_t *p, int32_t v, int n) {
   if (i < n) p[i] = v;
 }
 
+// Longest-first queue order for the next persistent launch: the simulations
+// of src[0..n) into dst[0..n) by decreasing ROI area (the cost of the next
+// steps is close to that of the last ones; before a simulation's first step
+// its occupied-list length stands in), finished simulations last.  A
+// counting sort over kOrderBuckets buckets of area / 8 in one workgroup;
+// the order within a bucket is arbitrary.  Each simulation's results do not
+// depend on when it runs.
+constexpr int kOrderBuckets = 1024;
+constexpr int kOrderThreads = 1024;
+
+__device__ __forceinline__ int order_bucket(const SimHdr &h, const SimDesc &d) {
+  if (h.step >= d.horizon) return 0;
+  const int area = h.roi[2] * h.roi[3];
+  return 1 + min((area > 0 ? area : h.n) >> 3, kOrderBuckets - 2);
+}
+
+__global__ __launch_bounds__(kOrderThreads) void order_by_cost_kernel(
+    const int32_t *src, int32_t *dst, int n, const SimHdr *hdr, const SimDesc *desc) {
+  __shared__ int32_t cnt[kOrderBuckets];
+  __shared__ int32_t wsum[kOrderThreads / 64];
+  const int tid = threadIdx.x;
+  cnt[tid] = 0;
+  __syncthreads();
+  for (int i = tid; i < n; i += kOrderThreads) {
+    const int sim = src[i];
+    atomicAdd(&cnt[order_bucket(hdr[sim], desc[sim])], 1);
+  }
+  __syncthreads();
+  // Exclusive prefix over buckets from the highest down: thread t holds
+  // bucket kOrderBuckets - 1 - t.
+  const int b = kOrderBuckets - 1 - tid;
+  const int c = cnt[b];
+  int incl = c;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o, 64);
+    if ((tid & 63) >= o) incl += v;
+  }
+  if ((tid & 63) == 63) wsum[tid >> 6] = incl;
+  __syncthreads();
+  int base = 0;
+  for (int w = 0; w < (tid >> 6); ++w) base += wsum[w];
+  __syncthreads();
+  cnt[b] = base + incl - c;
+  __syncthreads();
+  for (int i = tid; i < n; i += kOrderThreads) {
+    const int sim = src[i];
+    dst[atomicAdd(&cnt[order_bucket(hdr[sim], desc[sim])], 1)] = sim;
+  }
+}
+
 }  // namespace dev
 
 // ---------------------------------------------------------------------------
@@ -486,6 +536,14 @@ hipError_t launch_count_living(const SimDesc *desc, const uint8_t *st_pool, uint
                                int nsims, hipStream_t s) {
   hipLaunchKernelGGL(dev::count_living_kernel<256>, dim3(nsims), dim3(256), 0, s, desc, st_pool,
                      out, nsims);
+  return hipGetLastError();
+}
+
+hipError_t launch_order_by_cost(const int32_t *src, int32_t *dst, int n, const SimHdr *hdr,
+                                const SimDesc *desc, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(dev::order_by_cost_kernel, dim3(1), dim3(dev::kOrderThreads), 0, s, src, dst,
+                     n, hdr, desc);
   return hipGetLastError();
 }
 
