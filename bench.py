@@ -32,7 +32,7 @@ FP32_NONPACKED_TFLOPS = 78.6  # the same without packed FP32 (the kernels are co
 # HBM bytes per launch of the persistent kernel (variant 36) on this workload
 # (one hour of 20,000 simulations): rocprofv3 FETCH_SIZE + WRITE_SIZE, each
 # scaled by what the counter reads for a 1 GiB device copy (x2.0, x1.0).
-TRAFFIC_BYTES_PER_LAUNCH = 3.32e9 + 5.95e9
+TRAFFIC_BYTES_PER_LAUNCH = 3.42e9 + 7.40e9
 
 
 def parse():
