@@ -311,6 +311,8 @@ __global__ void fill_i32_kernel(int32_t *p, int32_t v, int n) {
 // depend on when it runs.
 constexpr int kOrderBuckets = 1024;
 constexpr int kOrderThreads = 1024;
+static_assert(kOrderThreads == kOrderBuckets && kOrderThreads % 64 == 0,
+              "one bucket per thread in the prefix pass");
 
 __device__ __forceinline__ int order_bucket(const SimHdr &h, const SimDesc &d) {
   if (h.step >= d.horizon) return 0;
