@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--sims", type=int, default=0,
                     help="use only the first N simulations (profiling runs)")
     ap.add_argument("--tune-steps", type=int, default=600)
+    ap.add_argument("--tune-refine-warm", type=int, default=1800,
+                    help="re-time the closest variants after this many steps (0: off)")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the surrogate training / screening sample")
     ap.add_argument("--surrogate-only", action="store_true",
@@ -174,7 +176,8 @@ def main():
     if args.variant < 0 and not args.no_tune:
         from emt6ro.tune import autotune
         tuned = autotune(b, {56: (53, 53)}, steps=args.tune_steps, use_cache=False,
-                         log=lambda m: print(m, file=sys.stderr, flush=True))
+                         log=lambda m: print(m, file=sys.stderr, flush=True),
+                         refine_warm=args.tune_refine_warm)
     for _ in range(args.warmup):
         b.run(hour, path="persistent", variant=args.variant)
     b.reset(0)

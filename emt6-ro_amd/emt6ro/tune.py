@@ -33,10 +33,18 @@ def _save(d):
         pass
 
 
-def autotune(batch, classes, steps=150, use_cache=True, log=None):
+def autotune(batch, classes, steps=150, use_cache=True, log=None, refine_warm=0,
+             refine_top=3, refine_within=0.05):
     """classes: {rs: (hmax, wmax)} of the batch's size classes.
     Returns {rs: {"variant": id, "ms": {id: ms}}}; the batch is left reset
-    (epoch 0) with the chosen variants set."""
+    (epoch 0) with the chosen variants set.
+
+    With refine_warm > 0, the (at most refine_top) variants within
+    refine_within of the fastest are timed again on `steps` steps that start
+    after refine_warm steps run with the fastest one, since tumours grow and
+    the ranking of close variants can change with the ROI size; results do
+    not depend on the variant, so every candidate starts from the same
+    state."""
     cache = _load() if use_cache else {}
     out = {}
     vs = variants()
@@ -56,8 +64,23 @@ def autotune(batch, classes, steps=150, use_cache=True, log=None):
             if log:
                 log(f"[tune] rs={rs} variant={v} {timings[v['id']]:.2f} ms")
         best = min(timings, key=timings.get)
-        batch.set_variant(rs, best)
         out[rs] = {"variant": best, "ms": timings}
+        if refine_warm > 0:
+            close = sorted((t, v) for v, t in timings.items()
+                           if t <= timings[best] * (1.0 + refine_within))[:refine_top]
+            later = {}
+            for _, v in close:
+                batch.reset(0)
+                batch.set_variant(rs, best)
+                batch.run(refine_warm, path="persistent")
+                batch.set_variant(rs, v)
+                later[v] = batch.run(steps, path="persistent")
+                if log:
+                    log(f"[tune] rs={rs} variant {v} after {refine_warm} steps: {later[v]:.2f} ms")
+            best = min(later, key=later.get)
+            out[rs] = {"variant": best, "ms": timings, "ms_later": later,
+                       "refine_warm": refine_warm}
+        batch.set_variant(rs, best)
         cache[key] = out[rs]
     batch.reset(0)
     if use_cache:

@@ -106,3 +106,11 @@ def test_autotuner_keeps_results():
     assert set(res) == {56, 36}
     b.run(400)
     assert np.array_equal(b.results(), want)
+    # Re-timing the closest variants later in the run keeps results too.
+    b.reset(0)
+    res = autotune(b, {56: (53, 53)}, steps=60, use_cache=False, log=log, refine_warm=120,
+                   refine_top=2, refine_within=1.0)
+    later = res[56]["ms_later"]
+    assert 1 <= len(later) <= 2 and res[56]["variant"] in later
+    b.run(400)
+    assert np.array_equal(b.results(), want)
