@@ -227,6 +227,9 @@ def main():
         },
     }
     if not args.no_plain:
+        # Capture and instantiate the per-step graph outside the timed region.
+        b.reset(0)
+        b.run(128, path="graph", variant=args.variant)
         b.reset(0)
         t0 = time.perf_counter()
         b.run(hour, path="graph", variant=args.variant)

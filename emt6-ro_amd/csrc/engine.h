@@ -114,9 +114,12 @@ class Batch {
   int64_t sub_total_ = 0, st_total_ = 0, slot_total_ = 0;
   double last_ms_ = 0.0;
   int64_t last_launches_ = 0;
-  // captured per-step graph
+  // captured per-step graph and the launch arguments that can change between
+  // runs (epoch, instrumentation record): a change recaptures it
   hipGraphExec_t graph_exec_ = nullptr;
   int graph_steps_ = 0, graph_variant_ = -1;
+  uint32_t graph_epoch_ = 0;
+  unsigned long long *graph_instr_ = nullptr;
 };
 
 }  // namespace emt6ro

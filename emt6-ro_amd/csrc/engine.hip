@@ -419,7 +419,8 @@ void Batch::run(uint32_t nsteps, int path, int variant, uint32_t steps_per_launc
     uint32_t done = 0;
     if (path == kPathStepGraph) {
       const int K = 128;
-      if (!graph_exec_ || graph_variant_ != variant || graph_steps_ != K) {
+      if (!graph_exec_ || graph_variant_ != variant || graph_steps_ != K ||
+          graph_epoch_ != a.epoch || graph_instr_ != a.instr) {
         if (graph_exec_) EMT_HIP(hipGraphExecDestroy(graph_exec_));
         graph_exec_ = nullptr;
         hipGraph_t g = nullptr;
@@ -430,6 +431,8 @@ void Batch::run(uint32_t nsteps, int path, int variant, uint32_t steps_per_launc
         EMT_HIP(hipGraphDestroy(g));
         graph_variant_ = variant;
         graph_steps_ = K;
+        graph_epoch_ = a.epoch;
+        graph_instr_ = a.instr;
       }
       while (nsteps - done >= (uint32_t)K) {
         EMT_HIP(hipGraphLaunch(graph_exec_, stream_));

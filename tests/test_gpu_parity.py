@@ -116,6 +116,26 @@ def dose_table():
     return d
 
 
+def test_graph_follows_epoch_change(sim, params):
+    """The captured per-step graph holds the epoch among its launch arguments:
+    after a reset to another epoch the graph path must match the persistent
+    path at that epoch, not replay the old one."""
+    b, _ = build_pair(sim, params, dose_table(), reps=2, seed=7)
+    log("[parity] graph epoch change: capture at epoch 0, run at epoch 1")
+    b.run(512, path="graph")
+    ep0 = [np.array(b.state(i).cho()) for i in range(b.n)]
+    b.reset(1)
+    b.run(512, path="graph")
+    got = b.results().copy()
+    got_cho = [np.array(b.state(i).cho()) for i in range(b.n)]
+    b.reset(1)
+    b.run(512, path="persistent")
+    assert np.array_equal(b.results(), got)
+    for i in range(b.n):
+        assert np.array_equal(np.array(b.state(i).cho()), got_cho[i]), i
+    assert any(not np.array_equal(a, c) for a, c in zip(ep0, got_cho))
+
+
 @pytest.mark.parametrize("path", ["persistent", "step", "graph"])
 def test_paths_match_oracle_bitexact(sim, params, path):
     nsteps = {"persistent": 3000, "step": 400, "graph": 300}[path]
