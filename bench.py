@@ -29,10 +29,11 @@ METRIC = "simulated tumour-hours/sec on 1 MI355X; GA evals/sec surrogate vs plai
 FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 peak
 FLOP_PER_SITE_SUBSTEP = 36    # 3 substrates x (6 adds + 3 fma x 2)
 FP32_NONPACKED_TFLOPS = 78.6  # the same without packed FP32 (the kernels are compiled without it)
-# HBM bytes per launch of the persistent kernel (variant 36) on this workload
-# (one hour of 20,000 simulations): rocprofv3 FETCH_SIZE + WRITE_SIZE, each
-# scaled by what the counter reads for a 1 GiB device copy (x2.0, x1.0).
-TRAFFIC_BYTES_PER_LAUNCH = 3.42e9 + 7.40e9
+# HBM bytes per launch of the persistent kernel (variant 32, the tuned one) on
+# this workload (one hour of 20,000 simulations), averaged over the 146
+# launches of a 2 + 144 h run: rocprofv3 FETCH_SIZE + WRITE_SIZE, each scaled
+# by what the counter reads for a 1 GiB device copy (x2.0, x1.0).
+TRAFFIC_BYTES_PER_LAUNCH = 3.41e9 + 10.92e9
 
 
 def parse():
