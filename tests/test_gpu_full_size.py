@@ -1,0 +1,85 @@
+"""The benchmark's full workload on the GPU: 10 synthetic 53x53 tumours x 100
+random protocols x 20 replicates = 20,000 simulations, one simulated hour
+(600 steps, doses at 0 and 0.5 h included).
+
+At this size the checks are the ones that do not need the oracle on every
+simulation: the persistent kernel (lane-run diffusion) and the per-step
+kernels (banded diffusion, replayed from a HIP graph) are two independent
+implementations of the step, so their living counts, list lengths, division
+counts and sampled states must agree bit for bit; and simulations spread over
+the whole batch (the first and the last, and others at different protocol,
+tumour and replicate positions) match the CPU oracle bit for bit."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import golden
+from test_gpu_parity import compare_states, oracle_state_from_arrays
+
+pytestmark = pytest.mark.gpu
+
+SAMPLE = (0, 1, 19, 20, 199, 7777, 12345, 19980, 19999)
+FLOAT_FIELDS = ("cho", "ox", "gi", "time_in_repair", "irradiation", "repair_delay",
+                "proliferation_time", "cycle_g1", "cycle_s", "cycle_g2", "cycle_m", "cycle_d")
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def arrays_of(state):
+    out = {name: np.array(state.field(name)) for name in FLOAT_FIELDS}
+    out["occupancy"] = np.array(state.occupancy())
+    out["mode"] = np.array(state.mode())
+    out["phase"] = np.array(state.phase())
+    return out
+
+
+def test_full_workload_first_hour():
+    import emt6ro.simulation as sim
+    from emt6ro.batch import Batch
+    from emt6ro.workloads import batched_uniform
+    params = sim.load_parameters(golden("default-parameters.json"))
+    v = params.values()
+    wl = batched_uniform(params, seed=0)
+    n = len(wl["tumor_index"])
+    assert n == 20000
+    b = Batch(params, wl["tumors"], wl["tumor_index"], wl["prot_index"], wl["horizon"],
+              wl["doses"], 300, wl["prot_length"], sim_id=np.arange(n), seed=0)
+    log("[full] persistent, 600 steps, 20000 simulations")
+    b.run(600, path="persistent")
+    res_p = b.results().astype(np.int64)
+    cnt_p = b.counters()
+    assert (cnt_p["step"] == 600).all() and (cnt_p["status"] == 0).all()
+    states_p = {i: arrays_of(b.state(i)) for i in SAMPLE}
+
+    # Add the first simulation of a protocol dosing at 0 h and of one at 0.5 h.
+    prot = np.asarray(wl["prot_index"])
+    extra = [int(np.flatnonzero(prot == int(np.flatnonzero(wl["doses"][:, s] > 0)[0]))[0])
+             for s in (0, 1)]
+    log(f"[full] oracle on sampled simulations {list(SAMPLE) + extra}")
+    for i in list(SAMPLE) + extra:
+        t = int(wl["tumor_index"][i])
+        orc = O.OracleBatch(v, [oracle_state_from_arrays(arrays_of(wl["tumors"][t]), v)],
+                            wl["doses"], [int(wl["prot_index"][i])], [i], 300,
+                            wl["prot_length"])
+        orc.run(600, seed=0, nthreads=1)
+        assert res_p[i] == orc.counts()[0], (i, res_p[i], orc.counts()[0])
+        compare_states(b.state(i), orc, 0, f"full-size sim {i}")
+
+    log("[full] per-step graph path, 600 steps")
+    b.reset(0)
+    b.run(600, path="graph")
+    assert np.array_equal(b.results().astype(np.int64), res_p)
+    cnt_g = b.counters()
+    for k in ("step", "n_list", "divisions", "status"):
+        assert np.array_equal(cnt_g[k], cnt_p[k]), k
+    for i in SAMPLE:
+        got = arrays_of(b.state(i))
+        for name, a in states_p[i].items():
+            assert np.array_equal(got[name].view(np.uint8), a.view(np.uint8)), (i, name)
+    log(f"[full] mean living {res_p.mean():.2f}, divisions {int(cnt_p['divisions'].sum())}")
+    assert cnt_p["divisions"].sum() > 0
