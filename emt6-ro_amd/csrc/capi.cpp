@@ -30,8 +30,15 @@ int guarded(F &&f) {
   } catch (const emt6ro::Error &e) {
     g_last_error = e.what();
     return e.code();
+  } catch (const std::bad_alloc &) {
+    g_last_error = "Check failed: host allocation; out of host memory";
+    return EMT6RO_ERR_ARG;
   } catch (const std::exception &e) {
     g_last_error = e.what();
+    return EMT6RO_ERR_ARG;
+  } catch (...) {
+    // Nothing may unwind through the C ABI.
+    g_last_error = "Check failed: no exception escapes the C ABI; unknown exception";
     return EMT6RO_ERR_ARG;
   }
 }
