@@ -107,7 +107,8 @@ def test_gpu_diffusion_within_1e5_of_fp64(sim, params):
     """One step with zero metabolism changes substrates by diffusion only;
     the GPU's FP32 result must be within 1e-5 (relative to the external
     level) of the FP64 oracle on fixed states: 10 library tumours and
-    synthetic lattices of 33, 53 and 81 with random substrate fields."""
+    90 synthetic lattices of 33, 53 and 81 with random substrate fields
+    (100 fixed states)."""
     from emt6ro.batch import Batch
     from emt6ro.simulation import Parameters
     from emt6ro.synthetic import disk_state
@@ -118,7 +119,7 @@ def test_gpu_diffusion_within_1e5_of_fp64(sim, params):
     rng = np.random.default_rng(0)
     for t in range(1, 11):
         cases.append(sim.load_state(golden("tumor-lib", f"tumor-{t}.txt"), p0))
-    for k in range(30):
+    for k in range(90):
         size = (33, 53, 81)[k % 3]
         st, _ = disk_state(p0, size, size, radius=5 + k % 9, seed=k)
         for j, name in enumerate(("cho", "ox", "gi")):

@@ -83,3 +83,52 @@ def test_full_workload_first_hour():
             assert np.array_equal(got[name].view(np.uint8), a.view(np.uint8)), (i, name)
     log(f"[full] mean living {res_p.mean():.2f}, divisions {int(cnt_p['divisions'].sum())}")
     assert cnt_p["divisions"].sum() > 0
+
+
+def test_uneven_workload_first_hour():
+    """50,000 simulations mixing 33x33, 53x53 and 81x81 lattices, horizons of
+    48, 144 and 240 h and 1-20 replicates per (tumour, protocol): one
+    persistent launch per size class.  Over the first hour the persistent and
+    per-step paths agree bit for bit, and sampled simulations of every size
+    match the oracle."""
+    import emt6ro.simulation as sim
+    from emt6ro.batch import Batch
+    from emt6ro.workloads import batched_uneven
+    params = sim.load_parameters(golden("default-parameters.json"))
+    v = params.values()
+    wl = batched_uneven(params, seed=0)
+    n = len(wl["tumor_index"])
+    assert n == 50000 and set(np.unique(wl["sizes"]).tolist()) == {33, 53, 81}
+    b = Batch(params, wl["tumors"], wl["tumor_index"], wl["prot_index"], wl["horizon"],
+              wl["doses"], 300, wl["prot_length"], sim_id=np.arange(n), seed=3)
+    log("[full] uneven: persistent, 600 steps, 50000 simulations")
+    b.run(600, path="persistent")
+    res_p = b.results().astype(np.int64)
+    cnt_p = b.counters()
+    assert (cnt_p["step"] == 600).all() and (cnt_p["status"] == 0).all()
+    sample = [0, n - 1]
+    for size in (33, 53, 81):
+        idx = np.flatnonzero(wl["sizes"] == size)
+        sample += [int(idx[0]), int(idx[len(idx) // 2])]
+    sample = sorted(set(sample))
+    states_p = {i: arrays_of(b.state(i)) for i in sample}
+    log(f"[full] uneven: oracle on {sample}")
+    for i in sample:
+        t = int(wl["tumor_index"][i])
+        orc = O.OracleBatch(v, [oracle_state_from_arrays(arrays_of(wl["tumors"][t]), v)],
+                            wl["doses"], [int(wl["prot_index"][i])], [i], 300,
+                            wl["prot_length"])
+        orc.run(600, seed=3, nthreads=1)
+        assert res_p[i] == orc.counts()[0], (i, res_p[i], orc.counts()[0])
+        compare_states(b.state(i), orc, 0, f"uneven full-size sim {i} ({wl['sizes'][i]})")
+    log("[full] uneven: per-step graph path, 600 steps")
+    b.reset(0)
+    b.run(600, path="graph")
+    assert np.array_equal(b.results().astype(np.int64), res_p)
+    cnt_g = b.counters()
+    for k in ("step", "n_list", "divisions", "status"):
+        assert np.array_equal(cnt_g[k], cnt_p[k]), k
+    for i in sample:
+        got = arrays_of(b.state(i))
+        for name, a in states_p[i].items():
+            assert np.array_equal(got[name].view(np.uint8), a.view(np.uint8)), (i, name)
