@@ -435,7 +435,9 @@ hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds
   X(38, 3, 256, 2, 2, 2, 56, 3)   \
   X(39, 3, 512, 3, 4, 2, 88, 2)   \
   X(40, 3, 512, 3, 2, 2, 88, 2)   \
-  X(41, 3, 256, 2, 4, 2, 36, 4)
+  X(41, 3, 256, 2, 4, 2, 36, 4)   \
+  X(42, 3, 128, 4, 4, 3, 36, 4)   \
+  X(43, 3, 128, 2, 2, 3, 36, 4)
 
 const VariantInfo *variant_table(int *count) {
   static const VariantInfo table[] = {

@@ -208,6 +208,39 @@ def test_every_variant_matches(sim, params):
                 assert np.array_equal(a, bb), f"variant {v} differs"
 
 
+def test_every_variant_matches_oracle_on_33x33(sim, params):
+    """Every variant that fits 33x33 lattices (the stride-36 ones, among them
+    the 128- and 64-thread lane-run kernels, and the wider ones) against the
+    oracle bit for bit."""
+    from emt6ro.batch import Batch, variants, fits
+    from emt6ro.synthetic import disk_state
+    v = params.values()
+    tumors, orc_states = [], []
+    for k, r in enumerate((5, 7, 9, 11)):
+        st, arr = disk_state(params, 33, 33, radius=r, seed=300 + k)
+        tumors.append(st)
+        orc_states.append(oracle_state_from_arrays(arr, v))
+    d = dose_table()
+    n = len(tumors)
+    steps = 500
+    orc = O.OracleBatch(v, orc_states, d, np.arange(n) % d.shape[0], np.arange(n), 300, 144000)
+    orc.run(steps, seed=11, nthreads=4)
+    tested = 0
+    for var in variants():
+        if not fits(var, 33, 33):
+            continue
+        log(f"[parity] 33x33 variant {var}")
+        b = Batch(params, tumors, np.arange(n), np.arange(n) % d.shape[0],
+                  np.full(n, 2 ** 32 - 1, np.uint32), d, 300, 144000, sim_id=np.arange(n),
+                  seed=11)
+        b.run(steps, variant=var["id"])
+        assert np.array_equal(b.results().astype(np.int64), orc.counts()), var
+        for i in range(n):
+            compare_states(b.state(i), orc, i, f"33x33 variant {var['id']} sim {i}")
+        tested += 1
+    assert tested >= 10
+
+
 def test_reset_restores_initial_state(sim, params):
     b, _ = build_pair(sim, params, dose_table(), reps=1, seed=1)
     r0 = b.results()
