@@ -414,6 +414,13 @@ int emt6ro_batch_set_variant(emt6ro_batch *b, int32_t rs, int32_t variant) {
   });
 }
 
+int emt6ro_batch_set_window(emt6ro_batch *b, int32_t rows) {
+  return guarded([&] {
+    need(b != nullptr, "null pointer");
+    b->b->set_window(rows);
+  });
+}
+
 int emt6ro_batch_set_doses(emt6ro_batch *b, const float *prot_doses) {
   return guarded([&] {
     need(b && prot_doses, "null pointer");

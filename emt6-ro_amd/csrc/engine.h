@@ -65,6 +65,10 @@ class Batch {
   // Preferred variant per LDS row stride class (-1: choose_variant).
   void set_preferred_variant(int rs, int variant);
   int class_count() const { return (int)classes_.size(); }
+  // Window pass of the persistent path (lattices wider than row stride 36):
+  // rows of the window (0: off, -1: the most that keep eight workgroups on
+  // a CU).
+  void set_window(int rows);
 
  private:
   void sync();
@@ -98,6 +102,15 @@ class Batch {
   float *prot_ = nullptr;
   int32_t *order_ = nullptr;
   int32_t *order_next_ = nullptr;  // the order of the next launch, sorted by cost
+  int32_t *order_win_ = nullptr;   // the window kernel's queue
+  int32_t *wcount_ = nullptr;      // its length, per class
+  // Fraction of each class the window queue held at the end of the last
+  // run; below kWindowMin the window kernel is skipped (its launch would be
+  // mostly tail).  Reset to 1 by reset().
+  std::vector<double> win_frac_;
+  static constexpr double kWindowMin = 0.4;
+  uint32_t *target_ = nullptr;     // per-simulation target step of a launch
+  int window_rows_ = -1;
   int32_t *queue_ = nullptr;
   uint32_t *results_ = nullptr;
   int32_t *scratch_ = nullptr;

@@ -174,6 +174,10 @@ Batch::Batch(const HostParams &p, const std::vector<const TumorState *> &tumors,
   prot_ = dalloc<float>((size_t)spec.n_prot * spec.n_slots);
   order_ = dalloc<int32_t>(nsims_);
   order_next_ = dalloc<int32_t>(nsims_);
+  order_win_ = dalloc<int32_t>(nsims_);
+  wcount_ = dalloc<int32_t>(classes_.size());
+  win_frac_.assign(classes_.size(), 1.0);
+  target_ = dalloc<uint32_t>(nsims_);
   queue_ = dalloc<int32_t>(1);
   results_ = dalloc<uint32_t>(nsims_);
   EMT_HIP(hipMemcpyAsync(desc_, desc_h_.data(), nsims_ * sizeof(SimDesc), hipMemcpyHostToDevice, stream_));
@@ -209,7 +213,7 @@ Batch::~Batch() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
   dfree(desc_); dfree(hdr_); dfree(sub_); dfree(st_); dfree(slots_); dfree(prot_);
-  dfree(order_); dfree(order_next_); dfree(queue_); dfree(results_); dfree(scratch_); dfree(ovf_);
+  dfree(order_); dfree(order_next_); dfree(order_win_); dfree(wcount_); dfree(target_); dfree(queue_); dfree(results_); dfree(scratch_); dfree(ovf_);
   for (auto &p : img_sub_) dfree(p);
   for (auto &p : img_st_) dfree(p);
   for (auto &p : img_slots_) dfree(p);
@@ -231,6 +235,7 @@ void Batch::set_protocols(const float *doses) {
 void Batch::reset(uint32_t epoch) {
   EMT_HIP(hipSetDevice(device_));
   epoch_ = epoch;
+  std::fill(win_frac_.begin(), win_frac_.end(), 1.0);
   hipLaunchKernelGGL(replicate_kernel, dim3(nsims_), dim3(256), 0, stream_, desc_, tumor_of_,
                      img_desc_, img_hdr_d_, img_sub_[0], img_st_[0], img_slots_[0], sub_, st_,
                      slots_, hdr_, nsims_);
@@ -355,6 +360,12 @@ int Batch::choose_variant(int hmax, int wmax, int kind) {
   return best;
 }
 
+void Batch::set_window(int rows) {
+  EMT_ENFORCE(rows == -1 || rows == 0 || rows >= 8, -1,
+              "Window rows must be -1 (automatic), 0 (off) or at least 8; got ", rows);
+  window_rows_ = rows;
+}
+
 void Batch::set_preferred_variant(int rs, int variant) {
   const int k = rs <= 36 ? 0 : rs <= 56 ? 1 : 2;
   preferred_[k] = variant;
@@ -385,6 +396,31 @@ void Batch::run(uint32_t nsteps, int path, int variant, uint32_t steps_per_launc
       plans.push_back(Plan{v, grid, lds});
       max_grid = std::max(max_grid, grid);
     }
+    // Window pass: for classes wider than row stride 36 (and the default
+    // variant choice), simulations whose ROI fits a window first run in the
+    // window kernel; the class's own launch then takes every simulation to
+    // the same per-simulation target.
+    struct WinPlan { int rows, grid; size_t lds; };
+    std::vector<WinPlan> wins(classes_.size(), WinPlan{0, 0, 0});
+    if (variant < 0 && window_rows_ != 0) {
+      for (size_t ci = 0; ci < classes_.size(); ++ci) {
+        const Class &c = classes_[ci];
+        if (c.rs <= 36) continue;
+        int rows = window_rows_;
+        if (rows < 0) {
+          const size_t fixed = window_lds_bytes(0, c.hmax, c.wmax);
+          const size_t budget = (size_t)163840 / 8;
+          rows = budget > fixed ? (int)((budget - fixed) / (3 * 36 * sizeof(float))) : 0;
+        }
+        rows = std::min(rows, c.hmax + 2);
+        if (rows < 8) continue;
+        const size_t lds = window_lds_bytes(rows, c.hmax, c.wmax);
+        const int occ = window_occupancy(lds);
+        if (occ <= 0) continue;
+        wins[ci] = WinPlan{rows, std::min(c.count, occ * dev_cus), lds};
+        max_grid = std::max(max_grid, wins[ci].grid);
+      }
+    }
     ensure_scratch(max_grid);
     EMT_HIP(hipEventRecord(ev0_, stream_));
     uint32_t done = 0;
@@ -394,6 +430,25 @@ void Batch::run(uint32_t nsteps, int path, int variant, uint32_t steps_per_launc
       for (size_t ci = 0; ci < classes_.size(); ++ci) {
         const Class &c = classes_[ci];
         dev::RunArgs a = args(k);
+        if (wins[ci].rows > 0) {
+          const WinPlan &w = wins[ci];
+          EMT_HIP(launch_set_target(order_ + c.offset, c.count, hdr_, desc_, target_, k, stream_));
+          EMT_HIP(launch_order_window(order_ + c.offset, order_win_ + c.offset, c.count, hdr_,
+                                      desc_, target_, w.rows, wcount_ + ci, stream_));
+          if (win_frac_[ci] >= kWindowMin) {
+            dev::RunArgs aw = args(k);
+            aw.order = order_win_ + c.offset;
+            aw.nsims = c.count;
+            aw.nsims_dev = wcount_ + ci;
+            aw.target = target_;
+            aw.pv_max = w.rows * 36;
+            EMT_HIP(launch_fill_i32(queue_, 0, 1, stream_));
+            EMT_HIP(launch_window(aw, w.grid, w.lds, stream_));
+            launches += 2;
+          }
+          a.target = target_;
+          launches += 2;
+        }
         // Longest first by the ROI each simulation reached.
         EMT_HIP(launch_order_by_cost(order_ + c.offset, order_next_ + c.offset, c.count, hdr_,
                                      desc_, stream_));
@@ -407,6 +462,18 @@ void Batch::run(uint32_t nsteps, int path, int variant, uint32_t steps_per_launc
       done += k;
     }
     EMT_HIP(hipEventRecord(ev1_, stream_));
+    // Eligible fraction of each windowed class, for the next run's choice.
+    bool any_win = false;
+    for (const WinPlan &w : wins) any_win = any_win || w.rows > 0;
+    if (any_win) {
+      std::vector<int32_t> wc(classes_.size(), 0);
+      EMT_HIP(hipMemcpyAsync(wc.data(), wcount_, wc.size() * sizeof(int32_t),
+                             hipMemcpyDeviceToHost, stream_));
+      sync();
+      for (size_t ci = 0; ci < classes_.size(); ++ci)
+        if (wins[ci].rows > 0 && classes_[ci].count > 0)
+          win_frac_[ci] = (double)wc[ci] / classes_[ci].count;
+    }
   } else if (path == kPathStep || path == kPathStepGraph) {
     // The per-step kernels use the banded scheme.
     if (variant < 0 || vt[variant].kind != 0) variant = choose_variant(hmax_, wmax_, 0);

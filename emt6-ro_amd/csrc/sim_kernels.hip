@@ -75,7 +75,8 @@ __global__ __launch_bounds__(NT, WPE) void persistent_kernel(RunArgs a) {
     SimHdr *hd = a.hdr + sim;
     const uint32_t step0 = hd->step;
     const uint32_t left = d.horizon > step0 ? d.horizon - step0 : 0u;
-    const uint32_t nsteps = a.nsteps < left ? a.nsteps : left;
+    const uint32_t nsteps = a.target ? (a.target[sim] > step0 ? a.target[sim] - step0 : 0u)
+                                     : (a.nsteps < left ? a.nsteps : left);
     if (nsteps == 0u) {
       __syncthreads();
       continue;
@@ -124,6 +125,194 @@ __global__ __launch_bounds__(NT, WPE) void persistent_kernel(RunArgs a) {
     const uint64_t tot = wg_reduce_sum_u64<NT>(updates64, sm.red);
     if (tid == 0) {
       hd->step = step0 + nsteps;
+      hd->n = sm.n;
+      hd->cur = sm.cur;
+      for (int q = 0; q < 5; ++q) hd->roi[q] = sm.roi[q];
+      hd->divisions += sm.divisions;
+      hd->status |= sm.status;
+      hd->site_substeps += tot;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Window kernel: the persistent kernel for lattices larger than its row
+// stride RS, with only a window of NR x RS padded sites of the substrates in
+// LDS (the site states stay whole).  The window holds the temporary grid of
+// the ROI (the ROI and two rings: padded rows r0-1 .. r0+h+2, columns
+// c0-1 .. c0+w+2); every substrate a step touches lies there: diffusion
+// stays in the temporary grid, and cells lie in the ROI of the last refresh
+// or, born since, one site outside it.  Window site (i, j) is padded site
+// (wr + i, wc + j).  The diffusion and cell phases take ROI coordinates
+// shifted into the window, so their addressing is the full-lattice one.
+// When a refreshed ROI leaves the window it moves; when it no longer fits,
+// the simulation stops before that step and a full-lattice launch takes it
+// to its target.
+// ---------------------------------------------------------------------------
+template <int RS>
+__device__ __forceinline__ bool window_origin(const int roi[5], int NR, int &wr, int &wc) {
+  const int h = roi[2], w = roi[3];
+  if (h == 0 || w == 0) { wr = 0; wc = 0; return true; }
+  if (h + 4 > NR || w + 4 > RS) return false;
+  wr = roi[0] - 1 - (NR - (h + 4)) / 2;
+  wc = roi[1] - 1 - (RS - (w + 4)) / 2;
+  return true;
+}
+
+template <int RS>
+__device__ __forceinline__ bool window_holds(const int roi[5], int NR, int wr, int wc) {
+  if (roi[2] == 0 || roi[3] == 0) return true;
+  return roi[0] - 1 >= wr && roi[0] + roi[2] + 2 < wr + NR && roi[1] - 1 >= wc &&
+         roi[1] + roi[3] + 2 < wc + RS;
+}
+
+template <int NT, int RS>
+__device__ void load_window(const RunArgs &a, const SimDesc &d, float *win, int PV, int wr, int wc) {
+  const int H = d.H, W = d.W, HW = H * W;
+  const float *g = a.sub_pool + d.sub_off;
+  for (int q = threadIdx.x; q < PV; q += NT) {
+    const int qr = q / RS, pr = wr + qr, pc = wc + (q - qr * RS);
+    const bool in = pr >= 1 && pr <= H && pc >= 1 && pc <= W;
+    const int x = (pr - 1) * W + (pc - 1);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) win[s * PV + q] = in ? g[s * HW + x] : 0.f;
+  }
+}
+
+template <int NT, int RS>
+__device__ void store_window(const RunArgs &a, const SimDesc &d, const float *win, int PV, int wr,
+                             int wc) {
+  const int H = d.H, W = d.W, HW = H * W;
+  float *g = a.sub_pool + d.sub_off;
+  for (int q = threadIdx.x; q < PV; q += NT) {
+    const int qr = q / RS, pr = wr + qr, pc = wc + (q - qr * RS);
+    if (pr >= 1 && pr <= H && pc >= 1 && pc <= W) {
+      const int x = (pr - 1) * W + (pc - 1);
+#pragma unroll
+      for (int s = 0; s < 3; ++s) g[s * HW + x] = win[s * PV + q];
+    }
+  }
+}
+
+// Substrates of lattice site (r, c) in the window.
+template <int RS>
+struct SubWin {
+  float *win;
+  int PV, wr, wc;
+  __device__ float *at(int s, int r, int c) const {
+    return win + s * PV + (r + 1 - wr) * RS + (c + 1 - wc);
+  }
+};
+
+template <int NT, int RS, typename Diff, int WPE>
+__global__ __launch_bounds__(NT, WPE) void window_kernel(RunArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  Shared &sm = *reinterpret_cast<Shared *>(smem);
+  float *win = reinterpret_cast<float *>(smem + sizeof(Shared));
+  const int PV = a.pv_max;
+  const int NR = PV / RS;
+  uint8_t *st = reinterpret_cast<uint8_t *>(win + Diff::kPlanes * PV);
+  int32_t *pfx = a.scratch + (int64_t)blockIdx.x * a.scratch_stride;
+  uint64_t *ovf = a.ovf + (int64_t)blockIdx.x * a.ovf_stride;
+  const int tid = threadIdx.x;
+  const int nq = *a.nsims_dev;
+  Diff diff;
+  for (;;) {
+    if (tid == 0) sm.sim = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int qi = sm.sim;
+    if (qi >= nq) break;
+    const int sim = a.order[qi];
+    const SimDesc d = a.desc[sim];
+    SimHdr *hd = a.hdr + sim;
+    const uint32_t step0 = hd->step;
+    const uint32_t tgt = a.target[sim];
+    const uint32_t nsteps = tgt > step0 ? tgt - step0 : 0u;
+    int roi[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) roi[q] = hd->roi[q];
+    int wr = 0, wc = 0;
+    if (nsteps == 0u || !window_origin<RS>(roi, NR, wr, wc)) {
+      __syncthreads();
+      continue;
+    }
+    const int H = d.H, W = d.W, HW = H * W;
+    load_window<NT, RS>(a, d, win, PV, wr, wc);
+    {
+      const uint32_t *gs = reinterpret_cast<const uint32_t *>(a.st_pool + d.st_off);
+      uint32_t *ls = reinterpret_cast<uint32_t *>(st);
+      for (int k = tid; k < (HW + 3) / 4; k += NT) ls[k] = gs[k];
+    }
+    if (tid == 0) {
+      sm.n = hd->n;
+      sm.cur = hd->cur;
+      for (int k = 0; k < 5; ++k) sm.roi[k] = roi[k];
+      sm.divisions = 0u;
+      sm.status = 0u;
+    }
+    __syncthreads();
+    uint32_t *slots = a.slot_pool + d.slot_off;
+    const RngKey key{a.seed, a.epoch, d.sim_id};
+    uint32_t updates = 0u;
+    uint64_t updates64 = 0u;
+    int roiw[5] = {roi[0] - wr, roi[1] - wc, roi[2], roi[3], roi[4]};
+    diff.set_sim(qi);
+    diff.plan(roiw, sm);
+    uint32_t k = 0;
+    for (; k < nsteps; ++k) {
+      const uint32_t t = step0 + k;
+      // The ROI refresh comes before the list rebuild here (the rebuild does
+      // not change the site states, so the ROI is the same), so that a
+      // simulation leaving the window stops before any work of step t.
+      if (t % 32u == 0u) {
+        find_roi<NT>(st, H, W, sm);
+        int nr[5];
+        bool same = true;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          nr[q] = sm.roi[q];
+          same = same && nr[q] == roi[q];
+        }
+        if (!same) {
+          if (!window_holds<RS>(nr, NR, wr, wc)) {
+            int nwr, nwc;
+            if (!window_origin<RS>(nr, NR, nwr, nwc)) break;
+            store_window<NT, RS>(a, d, win, PV, wr, wc);
+            // The new window is loaded from what the old one just stored:
+            // the stores must be complete before any thread's loads.
+            __threadfence_block();
+            __syncthreads();
+            wr = nwr;
+            wc = nwc;
+            load_window<NT, RS>(a, d, win, PV, wr, wc);
+            __syncthreads();
+          }
+#pragma unroll
+          for (int q = 0; q < 5; ++q) roi[q] = nr[q];
+          roiw[0] = roi[0] - wr; roiw[1] = roi[1] - wc;
+          roiw[2] = roi[2]; roiw[3] = roi[3]; roiw[4] = roi[4];
+          diff.plan(roiw, sm);
+        }
+      }
+      if (t % 128u == 0u) rebuild_list<NT>(st, H, W, slots, d.cap, pfx, sm);
+      diff.run(a.p, win, PV, roiw, updates);
+      const float dose = dose_at(a, d.prot, t);
+      cells_step<NT>(a.p, key, t, dose, SubWin<RS>{win, PV, wr, wc}, st, H, W, slots, d.cap, ovf,
+                     sm, a.instr);
+      if ((k & 1023u) == 1023u) { updates64 += updates; updates = 0u; }
+    }
+    updates64 += updates;
+    __syncthreads();
+    store_window<NT, RS>(a, d, win, PV, wr, wc);
+    {
+      uint32_t *gs = reinterpret_cast<uint32_t *>(a.st_pool + d.st_off);
+      const uint32_t *ls = reinterpret_cast<const uint32_t *>(st);
+      for (int q = tid; q < (HW + 3) / 4; q += NT) gs[q] = ls[q];
+    }
+    const uint64_t tot = wg_reduce_sum_u64<NT>(updates64, sm.red);
+    if (tid == 0) {
+      hd->step = step0 + k;
       hd->n = sm.n;
       hd->cur = sm.cur;
       for (int q = 0; q < 5; ++q) hd->roi[q] = sm.roi[q];
@@ -354,6 +543,65 @@ __global__ __launch_bounds__(kOrderThreads) void order_by_cost_kernel(
   }
 }
 
+// Per-simulation target of a launch: k steps further, capped by the horizon.
+__global__ void set_target_kernel(const int32_t *sims, int n, const SimHdr *hdr,
+                                  const SimDesc *desc, uint32_t *target, uint32_t k) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int sim = sims[i];
+  const uint32_t step = hdr[sim].step, hz = desc[sim].horizon;
+  const uint32_t left = hz > step ? hz - step : 0u;
+  target[sim] = step + (k < left ? k : left);
+}
+
+// The window kernel's queue: the simulations short of their target whose
+// ROI's temporary grid fits an NR x RS window with a site to spare on each
+// side, longest first; the count goes to *count.
+__global__ __launch_bounds__(kOrderThreads) void order_window_kernel(
+    const int32_t *src, int32_t *dst, int n, const SimHdr *hdr, const SimDesc *desc,
+    const uint32_t *target, int nr, int rs, int32_t *count) {
+  __shared__ int32_t cnt[kOrderBuckets];
+  __shared__ int32_t wsum[kOrderThreads / 64];
+  const int tid = threadIdx.x;
+  auto bucket = [&](int sim) -> int {
+    const SimHdr &h = hdr[sim];
+    if (h.step >= target[sim]) return -1;
+    const int hh = h.roi[2], ww = h.roi[3];
+    // One site of room on each side for the ROI to grow during the launch.
+    if (hh != 0 && ww != 0 && (hh + 6 > nr || ww + 6 > rs)) return -1;
+    return order_bucket(h, desc[sim]);
+  };
+  cnt[tid] = 0;
+  __syncthreads();
+  for (int i = tid; i < n; i += kOrderThreads) {
+    const int b = bucket(src[i]);
+    if (b >= 0) atomicAdd(&cnt[b], 1);
+  }
+  __syncthreads();
+  const int b = kOrderBuckets - 1 - tid;
+  const int c = cnt[b];
+  int incl = c;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o, 64);
+    if ((tid & 63) >= o) incl += v;
+  }
+  if ((tid & 63) == 63) wsum[tid >> 6] = incl;
+  __syncthreads();
+  int base = 0;
+  for (int w = 0; w < (tid >> 6); ++w) base += wsum[w];
+  int total = 0;
+  for (int w = 0; w < kOrderThreads / 64; ++w) total += wsum[w];
+  __syncthreads();
+  cnt[b] = base + incl - c;
+  if (tid == 0) *count = total;
+  __syncthreads();
+  for (int i = tid; i < n; i += kOrderThreads) {
+    const int sim = src[i];
+    const int bk = bucket(sim);
+    if (bk >= 0) dst[atomicAdd(&cnt[bk], 1)] = sim;
+  }
+}
+
 }  // namespace dev
 
 // ---------------------------------------------------------------------------
@@ -548,6 +796,46 @@ hipError_t launch_order_by_cost(const int32_t *src, int32_t *dst, int n, const S
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(dev::order_by_cost_kernel, dim3(1), dim3(dev::kOrderThreads), 0, s, src, dst,
                      n, hdr, desc);
+  return hipGetLastError();
+}
+
+// The window kernel: 128 threads, row stride 36, lane-run diffusion with
+// band heights 4/6/8 (lowest that fits), 2 lane slots, 4 waves per SIMD.
+using WindowDiff = dev::LaneRunDiffuser<128, 36, 3, 2, 4, 6, 8, 1>;
+#define EMT6RO_WINDOW_KERNEL dev::window_kernel<128, 36, WindowDiff, 4>
+
+size_t window_lds_bytes(int rows, int hmax, int wmax) {
+  const size_t planes = 3 * (size_t)36 * rows * sizeof(float);
+  const size_t stb = ((size_t)hmax * wmax + 15) & ~(size_t)15;
+  const size_t slack = (size_t)8 * 36 * sizeof(float) + 16;
+  return shared_bytes() + planes + (stb > slack ? stb : slack);
+}
+
+int window_occupancy(size_t lds) {
+  int nb = 0;
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &nb, reinterpret_cast<const void *>(EMT6RO_WINDOW_KERNEL), 128, lds);
+  return e == hipSuccess ? nb : 0;
+}
+
+hipError_t launch_window(const dev::RunArgs &a, int grid, size_t lds, hipStream_t s) {
+  return launch_lds(EMT6RO_WINDOW_KERNEL, grid, 128, lds, s, a);
+}
+
+hipError_t launch_set_target(const int32_t *sims, int n, const SimHdr *hdr, const SimDesc *desc,
+                             uint32_t *target, uint32_t k, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(dev::set_target_kernel, dim3((n + 255) / 256), dim3(256), 0, s, sims, n,
+                     hdr, desc, target, k);
+  return hipGetLastError();
+}
+
+hipError_t launch_order_window(const int32_t *src, int32_t *dst, int n, const SimHdr *hdr,
+                               const SimDesc *desc, const uint32_t *target, int rows,
+                               int32_t *count, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(dev::order_window_kernel, dim3(1), dim3(dev::kOrderThreads), 0, s, src, dst,
+                     n, hdr, desc, target, rows, 36, count);
   return hipGetLastError();
 }
 

@@ -50,6 +50,8 @@ struct RunArgs {
   int64_t ovf_stride;
   int32_t pv_max;             // LDS floats per substrate plane
   unsigned long long *instr;  // instrumentation counters (null: off), kInstr* layout
+  const uint32_t *target;     // per-simulation step to reach (null: advance by nsteps)
+  const int32_t *nsims_dev;   // queue length in device memory (window kernel)
 };
 
 // Instrumentation record (common/instr.h:5-21): repair decisions, repair

@@ -69,6 +69,7 @@ def _load():
         "emt6ro_variant_info": ([i32, C.POINTER(i32)], C.c_int),
         "emt6ro_variant_occupancy": ([i32, i32, i32], C.c_int),
         "emt6ro_batch_set_variant": ([P, i32, i32], C.c_int),
+    "emt6ro_batch_set_window": ([P, i32], C.c_int),
         "emt6ro_batch_set_doses": ([P, f32p], C.c_int),
         "emt6ro_instr_read": ([C.POINTER(u64), C.POINTER(C.c_double)], C.c_int),
         "emt6ro_instr_reset": ([], C.c_int),

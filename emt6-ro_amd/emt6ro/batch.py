@@ -77,6 +77,12 @@ class Batch:
         row stride rs (36, 56 or 88); -1 restores the default."""
         check(lib.emt6ro_batch_set_variant(self._h, int(rs), int(variant)))
 
+    def set_window(self, rows=-1):
+        """Window pass for lattices wider than 34 sites: simulations whose ROI
+        plus two rings fits `rows` x 36 padded sites run first in a kernel
+        holding only that window in LDS.  -1: automatic, 0: off."""
+        check(lib.emt6ro_batch_set_window(self._h, int(rows)))
+
     def set_doses(self, doses):
         d = np.ascontiguousarray(doses, np.float32)
         check(lib.emt6ro_batch_set_doses(self._h, _p(d, C.c_float)))

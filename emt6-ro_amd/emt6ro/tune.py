@@ -46,6 +46,11 @@ def autotune(batch, classes, steps=150, use_cache=True, log=None, refine_warm=0,
     not depend on the variant, so every candidate starts from the same
     state."""
     cache = _load() if use_cache else {}
+    # The variants are timed on the whole-lattice kernel alone: the window
+    # pass (which takes the simulations with small ROIs first) stays off
+    # until the end.
+    if hasattr(batch, "set_window"):
+        batch.set_window(0)
     out = {}
     vs = variants()
     for rs, (hmax, wmax) in sorted(classes.items()):
@@ -83,6 +88,8 @@ def autotune(batch, classes, steps=150, use_cache=True, log=None, refine_warm=0,
         batch.set_variant(rs, best)
         cache[key] = out[rs]
     batch.reset(0)
+    if hasattr(batch, "set_window"):
+        batch.set_window(-1)
     if use_cache:
         _save(cache)
     return out

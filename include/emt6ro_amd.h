@@ -173,6 +173,15 @@ int emt6ro_variant_info(int32_t idx, int32_t *info);
 /* Variant the persistent path uses for simulations whose lattice needs LDS
  * row stride `rs` (36, 56 or 88); -1 restores the default choice. */
 int emt6ro_batch_set_variant(emt6ro_batch *b, int32_t rs, int32_t variant);
+/* Window pass of the persistent path, no reference counterpart (the
+ * reference keeps every lattice whole, diffusion.cu:119-200): for lattices
+ * wider than 34 sites, simulations whose ROI plus two rings fits a window of
+ * `rows` x 36 padded sites run first in a kernel that keeps only that window
+ * of the substrates in LDS, so more simulations fit a CU; the rest, and any
+ * whose ROI outgrows the window, continue in the whole-lattice kernel.
+ * Results do not depend on it.  rows: -1 automatic (the default), 0 off,
+ * otherwise at least 8.  Only runs with variant -1 use it. */
+int emt6ro_batch_set_window(emt6ro_batch *b, int32_t rows);
 /* Replaces the batch's dose table (n_prot x n_slots, as at creation). */
 int emt6ro_batch_set_doses(emt6ro_batch *b, const float *prot_doses);
 /* Workgroups per CU the persistent kernel of `variant` reaches for lattices

@@ -241,6 +241,26 @@ def test_every_variant_matches_oracle_on_33x33(sim, params):
     assert tested >= 10
 
 
+@pytest.mark.parametrize("rows", [-1, 24, 21])
+def test_window_pass_matches_oracle(sim, params, rows):
+    """The window pass (only the ROI's temporary grid of a 53x53 lattice in
+    LDS) with the automatic window and with windows so small that the ROIs
+    of the tumours leave them or outgrow them (the window moves, or the
+    simulation finishes its launch in the whole-lattice kernel), over
+    several launches."""
+    log(f"[parity] window rows={rows}")
+    b, orc = build_pair(sim, params, dose_table(), reps=2, seed=4242)
+    b.set_window(rows)
+    b.run(3000, steps_per_launch=700)
+    orc.run(3000, seed=4242, nthreads=8)
+    assert np.array_equal(b.results().astype(np.int64), orc.counts())
+    cnt = b.counters()
+    assert (cnt["step"] == 3000).all() and (cnt["status"] == 0).all()
+    assert np.array_equal(cnt["n_list"].astype(np.int64), orc.nlist.astype(np.int64))
+    for i in range(b.n):
+        compare_states(b.state(i), orc, i, f"window rows {rows} sim {i}")
+
+
 def test_reset_restores_initial_state(sim, params):
     b, _ = build_pair(sim, params, dose_table(), reps=1, seed=1)
     r0 = b.results()
