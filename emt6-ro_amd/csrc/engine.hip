@@ -175,8 +175,8 @@ Batch::Batch(const HostParams &p, const std::vector<const TumorState *> &tumors,
   order_ = dalloc<int32_t>(nsims_);
   order_next_ = dalloc<int32_t>(nsims_);
   order_win_ = dalloc<int32_t>(nsims_);
-  wcount_ = dalloc<int32_t>(classes_.size());
-  win_frac_.assign(classes_.size(), 1.0);
+  wcount_ = dalloc<int32_t>(classes_.size() * kWindowKinds);
+  win_frac_.assign(classes_.size() * kWindowKinds, 1.0);
   target_ = dalloc<uint32_t>(nsims_);
   queue_ = dalloc<int32_t>(1);
   results_ = dalloc<uint32_t>(nsims_);
@@ -400,25 +400,36 @@ void Batch::run(uint32_t nsteps, int path, int variant, uint32_t steps_per_launc
     // variant choice), simulations whose ROI fits a window first run in the
     // window kernel; the class's own launch then takes every simulation to
     // the same per-simulation target.
-    struct WinPlan { int rows, grid; size_t lds; };
-    std::vector<WinPlan> wins(classes_.size(), WinPlan{0, 0, 0});
+    // Window pass: for classes wider than the window kernel's row stride
+    // (36, eight workgroups per CU) and the default variant choice, the
+    // simulations whose ROI fits run first in the window kernel; the
+    // class's own launch then takes every simulation to the same
+    // per-simulation target.
+    struct WinPlan { int rs, rows, grid; size_t lds; };
+    static const int kWinRs[kWindowKinds] = {36};
+    static const int kWinPerCu[kWindowKinds] = {8};
+    std::vector<WinPlan> wins(classes_.size() * kWindowKinds, WinPlan{0, 0, 0, 0});
     if (variant < 0 && window_rows_ != 0) {
       for (size_t ci = 0; ci < classes_.size(); ++ci) {
         const Class &c = classes_[ci];
-        if (c.rs <= 36) continue;
-        int rows = window_rows_;
-        if (rows < 0) {
-          const size_t fixed = window_lds_bytes(0, c.hmax, c.wmax);
-          const size_t budget = (size_t)163840 / 8;
-          rows = budget > fixed ? (int)((budget - fixed) / (3 * 36 * sizeof(float))) : 0;
+        for (int wk = 0; wk < kWindowKinds; ++wk) {
+          const int rs = kWinRs[wk];
+          if (c.rs <= rs || (wk > 0 && window_rows_ > 0)) continue;
+          int rows = window_rows_;
+          if (rows < 0) {
+            const size_t fixed = window_lds_bytes(rs, 0, c.hmax, c.wmax);
+            const size_t budget = (size_t)163840 / kWinPerCu[wk];
+            rows = budget > fixed ? (int)((budget - fixed) / (3 * rs * sizeof(float))) : 0;
+          }
+          rows = std::min(rows, c.hmax + 2);
+          if (rows < 8) continue;
+          const size_t lds = window_lds_bytes(rs, rows, c.hmax, c.wmax);
+          const int occ = window_occupancy(rs, lds);
+          if (occ <= 0) continue;
+          WinPlan &w = wins[ci * kWindowKinds + wk];
+          w = WinPlan{rs, rows, std::min(c.count, occ * dev_cus), lds};
+          max_grid = std::max(max_grid, w.grid);
         }
-        rows = std::min(rows, c.hmax + 2);
-        if (rows < 8) continue;
-        const size_t lds = window_lds_bytes(rows, c.hmax, c.wmax);
-        const int occ = window_occupancy(lds);
-        if (occ <= 0) continue;
-        wins[ci] = WinPlan{rows, std::min(c.count, occ * dev_cus), lds};
-        max_grid = std::max(max_grid, wins[ci].grid);
       }
     }
     ensure_scratch(max_grid);
@@ -430,24 +441,30 @@ void Batch::run(uint32_t nsteps, int path, int variant, uint32_t steps_per_launc
       for (size_t ci = 0; ci < classes_.size(); ++ci) {
         const Class &c = classes_[ci];
         dev::RunArgs a = args(k);
-        if (wins[ci].rows > 0) {
-          const WinPlan &w = wins[ci];
+        bool any = false;
+        for (int wk = 0; wk < kWindowKinds; ++wk) any = any || wins[ci * kWindowKinds + wk].rows > 0;
+        if (any) {
           EMT_HIP(launch_set_target(order_ + c.offset, c.count, hdr_, desc_, target_, k, stream_));
-          EMT_HIP(launch_order_window(order_ + c.offset, order_win_ + c.offset, c.count, hdr_,
-                                      desc_, target_, w.rows, wcount_ + ci, stream_));
-          if (win_frac_[ci] >= kWindowMin) {
+          for (int wk = 0; wk < kWindowKinds; ++wk) {
+            const size_t wi = ci * kWindowKinds + wk;
+            const WinPlan &w = wins[wi];
+            if (w.rows == 0) continue;
+            EMT_HIP(launch_order_window(order_ + c.offset, order_win_ + c.offset, c.count, hdr_,
+                                        desc_, target_, w.rows, w.rs, wcount_ + wi, stream_));
+            launches += 1;
+            if (win_frac_[wi] < kWindowMin) continue;
             dev::RunArgs aw = args(k);
             aw.order = order_win_ + c.offset;
             aw.nsims = c.count;
-            aw.nsims_dev = wcount_ + ci;
+            aw.nsims_dev = wcount_ + wi;
             aw.target = target_;
-            aw.pv_max = w.rows * 36;
+            aw.pv_max = w.rows * w.rs;
             EMT_HIP(launch_fill_i32(queue_, 0, 1, stream_));
-            EMT_HIP(launch_window(aw, w.grid, w.lds, stream_));
+            EMT_HIP(launch_window(w.rs, aw, w.grid, w.lds, stream_));
             launches += 2;
           }
           a.target = target_;
-          launches += 2;
+          launches += 1;
         }
         // Longest first by the ROI each simulation reached.
         EMT_HIP(launch_order_by_cost(order_ + c.offset, order_next_ + c.offset, c.count, hdr_,
@@ -466,13 +483,13 @@ void Batch::run(uint32_t nsteps, int path, int variant, uint32_t steps_per_launc
     bool any_win = false;
     for (const WinPlan &w : wins) any_win = any_win || w.rows > 0;
     if (any_win) {
-      std::vector<int32_t> wc(classes_.size(), 0);
+      std::vector<int32_t> wc(wins.size(), 0);
       EMT_HIP(hipMemcpyAsync(wc.data(), wcount_, wc.size() * sizeof(int32_t),
                              hipMemcpyDeviceToHost, stream_));
       sync();
-      for (size_t ci = 0; ci < classes_.size(); ++ci)
-        if (wins[ci].rows > 0 && classes_[ci].count > 0)
-          win_frac_[ci] = (double)wc[ci] / classes_[ci].count;
+      for (size_t wi = 0; wi < wins.size(); ++wi)
+        if (wins[wi].rows > 0 && classes_[wi / kWindowKinds].count > 0)
+          win_frac_[wi] = (double)wc[wi] / classes_[wi / kWindowKinds].count;
     }
   } else if (path == kPathStep || path == kPathStepGraph) {
     // The per-step kernels use the banded scheme.

@@ -36,15 +36,16 @@ size_t cells_lds_bytes(int hmax, int wmax);
 hipError_t launch_persistent_lds(int variant, const dev::RunArgs &a, int grid, size_t lds,
                                  hipStream_t s);
 int persistent_occupancy(int variant, size_t lds);
-// Window kernel (lattices wider than its row stride 36, only the ROI's
-// temporary grid in LDS): LDS bytes for `rows` window rows, workgroups per CU.
-size_t window_lds_bytes(int rows, int hmax, int wmax);
-int window_occupancy(size_t lds);
-hipError_t launch_window(const dev::RunArgs &a, int grid, size_t lds, hipStream_t s);
+// Window kernel (row stride rs = 36, for lattices wider than that; only the
+// ROI's temporary grid in LDS): LDS bytes for `rows` window rows, workgroups
+// per CU, launch.
+size_t window_lds_bytes(int rs, int rows, int hmax, int wmax);
+int window_occupancy(int rs, size_t lds);
+hipError_t launch_window(int rs, const dev::RunArgs &a, int grid, size_t lds, hipStream_t s);
 hipError_t launch_set_target(const int32_t *sims, int n, const SimHdr *hdr, const SimDesc *desc,
                              uint32_t *target, uint32_t k, hipStream_t s);
 hipError_t launch_order_window(const int32_t *src, int32_t *dst, int n, const SimHdr *hdr,
-                               const SimDesc *desc, const uint32_t *target, int rows,
+                               const SimDesc *desc, const uint32_t *target, int rows, int rs,
                                int32_t *count, hipStream_t s);
 hipError_t launch_step(int variant, const dev::RunArgs &a, int nsims, int hmax, int wmax,
                        hipStream_t s);

@@ -801,25 +801,29 @@ hipError_t launch_order_by_cost(const int32_t *src, int32_t *dst, int n, const S
 
 // The window kernel: 128 threads, row stride 36, lane-run diffusion with
 // band heights 4/6/8 (lowest that fits), 2 lane slots, 4 waves per SIMD.
-using WindowDiff = dev::LaneRunDiffuser<128, 36, 3, 2, 4, 6, 8, 1>;
-#define EMT6RO_WINDOW_KERNEL dev::window_kernel<128, 36, WindowDiff, 4>
+// (A 44-column window at six workgroups per CU ran slower than the
+// whole-lattice kernel.)
+using WindowDiff36 = dev::LaneRunDiffuser<128, 36, 3, 2, 4, 6, 8, 1>;
+#define EMT6RO_WINDOW_KERNEL_36 dev::window_kernel<128, 36, WindowDiff36, 4>
 
-size_t window_lds_bytes(int rows, int hmax, int wmax) {
-  const size_t planes = 3 * (size_t)36 * rows * sizeof(float);
+size_t window_lds_bytes(int rs, int rows, int hmax, int wmax) {
+  const size_t planes = 3 * (size_t)rs * rows * sizeof(float);
   const size_t stb = ((size_t)hmax * wmax + 15) & ~(size_t)15;
-  const size_t slack = (size_t)8 * 36 * sizeof(float) + 16;
+  const size_t slack = (size_t)8 * rs * sizeof(float) + 16;
   return shared_bytes() + planes + (stb > slack ? stb : slack);
 }
 
-int window_occupancy(size_t lds) {
+int window_occupancy(int rs, size_t lds) {
   int nb = 0;
+  if (rs != 36) return 0;
   const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &nb, reinterpret_cast<const void *>(EMT6RO_WINDOW_KERNEL), 128, lds);
+      &nb, reinterpret_cast<const void *>(EMT6RO_WINDOW_KERNEL_36), 128, lds);
   return e == hipSuccess ? nb : 0;
 }
 
-hipError_t launch_window(const dev::RunArgs &a, int grid, size_t lds, hipStream_t s) {
-  return launch_lds(EMT6RO_WINDOW_KERNEL, grid, 128, lds, s, a);
+hipError_t launch_window(int rs, const dev::RunArgs &a, int grid, size_t lds, hipStream_t s) {
+  if (rs == 36) return launch_lds(EMT6RO_WINDOW_KERNEL_36, grid, 128, lds, s, a);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_set_target(const int32_t *sims, int n, const SimHdr *hdr, const SimDesc *desc,
@@ -831,11 +835,11 @@ hipError_t launch_set_target(const int32_t *sims, int n, const SimHdr *hdr, cons
 }
 
 hipError_t launch_order_window(const int32_t *src, int32_t *dst, int n, const SimHdr *hdr,
-                               const SimDesc *desc, const uint32_t *target, int rows,
+                               const SimDesc *desc, const uint32_t *target, int rows, int rs,
                                int32_t *count, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(dev::order_window_kernel, dim3(1), dim3(dev::kOrderThreads), 0, s, src, dst,
-                     n, hdr, desc, target, rows, 36, count);
+                     n, hdr, desc, target, rows, rs, count);
   return hipGetLastError();
 }
 

@@ -261,6 +261,33 @@ def test_window_pass_matches_oracle(sim, params, rows):
         compare_states(b.state(i), orc, i, f"window rows {rows} sim {i}")
 
 
+def test_wide_rois_beside_window_matches_oracle(sim, params):
+    """Tumours whose ROI is too wide for the window (or outgrows it during a
+    launch) next to small ones, on 53x53 lattices, over several launches."""
+    from emt6ro.batch import Batch
+    from emt6ro.synthetic import disk_state
+    v = params.values()
+    tumors, orc_states = [], []
+    for k, r in enumerate((6, 15, 16, 17)):
+        st, arr = disk_state(params, 53, 53, radius=r, seed=500 + k)
+        tumors.append(st)
+        orc_states.append(oracle_state_from_arrays(arr, v))
+    d = dose_table()
+    n = 2 * len(tumors)
+    ti = np.repeat(np.arange(len(tumors)), 2)
+    pi = np.arange(n) % d.shape[0]
+    b = Batch(params, tumors, ti, pi, np.full(n, 2 ** 32 - 1, np.uint32), d, 300, 144000,
+              sim_id=np.arange(n), seed=9)
+    orc = O.OracleBatch(v, [orc_states[t] for t in ti], d, pi, np.arange(n), 300, 144000)
+    log("[parity] wide ROIs beside the window: disks of radius 6, 15, 16, 17")
+    b.run(2400, steps_per_launch=600)
+    orc.run(2400, seed=9, nthreads=8)
+    assert np.array_equal(b.results().astype(np.int64), orc.counts())
+    for i in range(n):
+        compare_states(b.state(i), orc, i, f"wider window sim {i}")
+        log(f"  sim {i}: ROI {orc.roi[i].tolist()}")
+
+
 def test_reset_restores_initial_state(sim, params):
     b, _ = build_pair(sim, params, dose_table(), reps=1, seed=1)
     r0 = b.results()
