@@ -109,7 +109,7 @@ class Batch {
   // mostly tail).  Reset to 1 by reset().
   std::vector<double> win_frac_;
   static constexpr double kWindowMin = 0.4;
-  static constexpr int kWindowKinds = 1;   // window row stride 36
+  static constexpr int kWindowKinds = 2;   // window row strides 36 and 56
   uint32_t *target_ = nullptr;     // per-simulation target step of a launch
   int window_rows_ = -1;
   int32_t *queue_ = nullptr;

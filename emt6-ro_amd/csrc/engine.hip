@@ -400,14 +400,15 @@ void Batch::run(uint32_t nsteps, int path, int variant, uint32_t steps_per_launc
     // variant choice), simulations whose ROI fits a window first run in the
     // window kernel; the class's own launch then takes every simulation to
     // the same per-simulation target.
-    // Window pass: for classes wider than the window kernel's row stride
-    // (36, eight workgroups per CU) and the default variant choice, the
-    // simulations whose ROI fits run first in the window kernel; the
-    // class's own launch then takes every simulation to the same
-    // per-simulation target.
+    // Window passes: for classes wider than a window kernel's row stride
+    // (36: 128 threads, eight workgroups per CU; 56: 256 threads, four per
+    // CU, only for classes whose whole-lattice kernel needs the 88 stride)
+    // and the default variant choice, the simulations whose ROI fits run
+    // first in the window kernels, narrowest first; the class's own launch
+    // then takes every simulation to the same per-simulation target.
     struct WinPlan { int rs, rows, grid; size_t lds; };
-    static const int kWinRs[kWindowKinds] = {36};
-    static const int kWinPerCu[kWindowKinds] = {8};
+    static const int kWinRs[kWindowKinds] = {36, 56};
+    static const int kWinPerCu[kWindowKinds] = {8, 4};
     std::vector<WinPlan> wins(classes_.size() * kWindowKinds, WinPlan{0, 0, 0, 0});
     if (variant < 0 && window_rows_ != 0) {
       for (size_t ci = 0; ci < classes_.size(); ++ci) {

@@ -36,9 +36,9 @@ size_t cells_lds_bytes(int hmax, int wmax);
 hipError_t launch_persistent_lds(int variant, const dev::RunArgs &a, int grid, size_t lds,
                                  hipStream_t s);
 int persistent_occupancy(int variant, size_t lds);
-// Window kernel (row stride rs = 36, for lattices wider than that; only the
-// ROI's temporary grid in LDS): LDS bytes for `rows` window rows, workgroups
-// per CU, launch.
+// Window kernels (row stride rs = 36 or 56, for lattices wider than that;
+// only the ROI's temporary grid in LDS): LDS bytes for `rows` window rows,
+// workgroups per CU, launch.
 size_t window_lds_bytes(int rs, int rows, int hmax, int wmax);
 int window_occupancy(int rs, size_t lds);
 hipError_t launch_window(int rs, const dev::RunArgs &a, int grid, size_t lds, hipStream_t s);

@@ -799,12 +799,16 @@ hipError_t launch_order_by_cost(const int32_t *src, int32_t *dst, int n, const S
   return hipGetLastError();
 }
 
-// The window kernel: 128 threads, row stride 36, lane-run diffusion with
-// band heights 4/6/8 (lowest that fits), 2 lane slots, 4 waves per SIMD.
-// (A 44-column window at six workgroups per CU ran slower than the
-// whole-lattice kernel.)
+// The window kernels, lane-run diffusion with band heights 4/6/8 (lowest
+// that fits), 2 lane slots, 4 waves per SIMD: row stride 36 with 128 threads
+// (eight workgroups per CU), and row stride 56 with 256 threads (four per
+// CU, for lattices wider than 54 sites, whose whole-lattice kernel holds one
+// workgroup per CU).  (A 44-column window at six 128-thread workgroups per
+// CU ran slower than the 53x53 whole-lattice kernel.)
 using WindowDiff36 = dev::LaneRunDiffuser<128, 36, 3, 2, 4, 6, 8, 1>;
+using WindowDiff56 = dev::LaneRunDiffuser<256, 56, 2, 2, 4, 6, 8, 1>;
 #define EMT6RO_WINDOW_KERNEL_36 dev::window_kernel<128, 36, WindowDiff36, 4>
+#define EMT6RO_WINDOW_KERNEL_56 dev::window_kernel<256, 56, WindowDiff56, 4>
 
 size_t window_lds_bytes(int rs, int rows, int hmax, int wmax) {
   const size_t planes = 3 * (size_t)rs * rows * sizeof(float);
@@ -815,14 +819,19 @@ size_t window_lds_bytes(int rs, int rows, int hmax, int wmax) {
 
 int window_occupancy(int rs, size_t lds) {
   int nb = 0;
-  if (rs != 36) return 0;
-  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &nb, reinterpret_cast<const void *>(EMT6RO_WINDOW_KERNEL_36), 128, lds);
+  hipError_t e = hipErrorInvalidValue;
+  if (rs == 36)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &nb, reinterpret_cast<const void *>(EMT6RO_WINDOW_KERNEL_36), 128, lds);
+  else if (rs == 56)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &nb, reinterpret_cast<const void *>(EMT6RO_WINDOW_KERNEL_56), 256, lds);
   return e == hipSuccess ? nb : 0;
 }
 
 hipError_t launch_window(int rs, const dev::RunArgs &a, int grid, size_t lds, hipStream_t s) {
   if (rs == 36) return launch_lds(EMT6RO_WINDOW_KERNEL_36, grid, 128, lds, s, a);
+  if (rs == 56) return launch_lds(EMT6RO_WINDOW_KERNEL_56, grid, 256, lds, s, a);
   return hipErrorInvalidValue;
 }
 

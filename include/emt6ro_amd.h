@@ -176,7 +176,8 @@ int emt6ro_batch_set_variant(emt6ro_batch *b, int32_t rs, int32_t variant);
 /* Window pass of the persistent path, no reference counterpart (the
  * reference keeps every lattice whole, diffusion.cu:119-200): for lattices
  * wider than 34 sites, simulations whose ROI plus two rings fits a window of
- * `rows` x 36 padded sites run first in a kernel that keeps only that window
+ * `rows` x 36 padded sites (and, on lattices wider than 54 sites, an
+ * automatic second window of 56 columns) run first in a kernel that keeps only that window
  * of the substrates in LDS, so more simulations fit a CU; the rest, and any
  * whose ROI outgrows the window, continue in the whole-lattice kernel.
  * Results do not depend on it.  rows: -1 automatic (the default), 0 off,

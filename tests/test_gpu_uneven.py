@@ -114,3 +114,35 @@ def test_autotuner_keeps_results():
     assert 1 <= len(later) <= 2 and res[56]["variant"] in later
     b.run(400)
     assert np.array_equal(b.results(), want)
+
+
+def test_81x81_windows_match_oracle():
+    """81x81 lattices whose ROIs fit the 36-column window, the 56-column
+    window, or neither, over several one-hour launches."""
+    import emt6ro.simulation as sim
+    from emt6ro.batch import Batch
+    from emt6ro.synthetic import disk_state
+    params = sim.load_parameters(golden("default-parameters.json"))
+    v = params.values()
+    radii = [8, 17, 20, 23]
+    tumors, orc_states = [], []
+    for k, r in enumerate(radii):
+        st, arr = disk_state(params, 81, 81, radius=r, seed=90 + k)
+        tumors.append(st)
+        orc_states.append(oracle_state_from_arrays(arr, v))
+    n = len(tumors)
+    doses = np.zeros((1, 480), np.float32)
+    doses[0, 1] = 2.0
+    steps = 1800
+    b = Batch(params, tumors, np.arange(n), np.zeros(n, np.int64),
+              np.full(n, 2 ** 32 - 1, np.uint32), doses, 300, 144000, sim_id=np.arange(n),
+              seed=6)
+    log(f"[uneven] 81x81 windows: radii {radii}, {steps} steps")
+    b.run(steps, steps_per_launch=600)
+    got = b.results().astype(np.int64)
+    for i in range(n):
+        orc = O.OracleBatch(v, [orc_states[i]], doses, [0], [i], 300, 144000)
+        orc.run(steps, seed=6, nthreads=4)
+        assert got[i] == orc.counts()[0], (i, got[i], orc.counts()[0])
+        compare_states(b.state(i), orc, 0, f"81x81 window sim {i} (r {radii[i]})")
+        log(f"  sim {i}: ROI {orc.roi[0].tolist()} living {got[i]}")
