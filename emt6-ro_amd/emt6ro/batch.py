@@ -79,8 +79,10 @@ class Batch:
 
     def set_window(self, rows=-1):
         """Window pass for lattices wider than 34 sites: simulations whose ROI
-        plus two rings fits `rows` x 36 padded sites run first in a kernel
-        holding only that window in LDS.  -1: automatic, 0: off."""
+        plus two rings fits `rows` x 36 padded sites (and, on lattices wider
+        than 54 sites with rows=-1, a 56-column window) run first in a kernel
+        holding only that window in LDS.  -1: automatic, 0: off.  Results do
+        not depend on it."""
         check(lib.emt6ro_batch_set_window(self._h, int(rows)))
 
     def set_doses(self, doses):
