@@ -557,21 +557,42 @@ uint32_t eo_arbitration_key(uint32_t i) {
  * the state after every death of the step; at most one division per
  * lattice, won by the candidate whose list index i has the smallest i mod 512,
  * ties to the largest i (the reference's per-thread last-wins, lowest-thread
- * reduction); the child is appended to the list. */
+ * reduction); the child is appended to the list.
+ *
+ * `rules` switches two of these to the reference's literal behaviour, for
+ * A/B statistics only (the product implements rules = 0):
+ *   EO_RULE_Q3_EVERY_ENTRY  every list entry whose site is occupied steps it,
+ *                           so a site re-occupied since the last rebuild is
+ *                           stepped once per entry (simulation.cu:86-88 over
+ *                           the pushed child, :105);
+ *   EO_RULE_Q4_LIST_ORDER   a ready cell's target is tested for vacancy right
+ *                           after its own step, in list order. */
 static int eo_cells_step(const eo_params *p, int H, int W, float *sub, uint8_t *st,
                          eo_cell *cells, uint16_t *list, int32_t *pn, int cap,
-                         int32_t *owner, uint8_t *flags, uint32_t t, uint32_t sim,
-                         float dose, uint32_t seed, uint32_t epoch, uint64_t *instr) {
+                         int32_t *owner, uint8_t *flags, int32_t *seen, uint32_t t,
+                         uint32_t sim, float dose, uint32_t seed, uint32_t epoch,
+                         uint64_t *instr, int rules) {
   const int HW = H * W;
   const int n = *pn;
+  const int q3 = (rules & EO_RULE_Q3_EVERY_ENTRY) != 0;
+  const int q4 = (rules & EO_RULE_Q4_LIST_ORDER) != 0;
+  int best = -1;
+  uint32_t best_key = 0xffffffffu;
+  int best_child = -1;
   for (int i = 0; i < n; ++i) {
     int x = list[i];
-    int live = (st[x] == EO_OCCUPIED && owner[x] == i);
+    int live = (st[x] == EO_OCCUPIED && (q3 || owner[x] == i));
     flags[i] = live ? (uint8_t)(0x10 | eo_vacant_neighbours(W, st, x)) : 0;
+    seen[x] = 0;
   }
   for (int i = 0; i < n; ++i) {
     if (!(flags[i] & 0x10)) continue;
     int x = list[i];
+    /* Q3 (every entry): a second entry of a site re-occupied since the last
+     * rebuild steps the cell again, if it is still alive (site.h:65), with
+     * draws of its own (Philox index = how many entries of x came before). */
+    if (q3 && st[x] != EO_OCCUPIED) { flags[i] = 0; continue; }
+    const uint32_t occ = (uint32_t)seen[x]++;
     eo_cell *c = &cells[x];
     float s3[3] = {sub[x], sub[HW + x], sub[2 * HW + x]};
     int ready = 0;
@@ -585,7 +606,12 @@ static int eo_cells_step(const eo_params *p, int H, int W, float *sub, uint8_t *
       int changed = eo_progress_clock(c, p->time_step);
       int need_u = (c->t_rep > 0 || (c->R > 0 && changed)) &&
                    (c->t_rep + p->time_step / 3600.f >= c->delay);
-      float u = need_u ? eo_uniform(seed, epoch, sim, t, (uint32_t)x, EO_PURPOSE_REPAIR) : 1.f;
+      float u = 1.f;
+      if (need_u) {
+        uint32_t o[4];
+        eo_draw(seed, epoch, sim, t, (uint32_t)x, EO_PURPOSE_REPAIR, occ, o);
+        u = eo_u01(o[0]);
+      }
       alive = eo_try_repair_instr(c, changed, p, u, instr);
       if (alive) ready = (c->phase == 4);
     }
@@ -594,11 +620,23 @@ static int eo_cells_step(const eo_params *p, int H, int W, float *sub, uint8_t *
     sub[2 * HW + x] = s3[2];
     if (!alive) st[x] = EO_VACANT;
     flags[i] = ready ? 0x20 : 0;
+    if (ready && q4) {
+      /* Q4 (list order): the target's vacancy is read right after this
+       * entry's step, as the reference's thread reads it
+       * (simulation.cu:88-93): deaths of earlier entries are seen, later
+       * ones are not. */
+      uint32_t o[4];
+      eo_draw(seed, epoch, sim, t, (uint32_t)x, EO_PURPOSE_NEIGHBOUR, occ, o);
+      int cr, cc;
+      eo_choose_neighbour(x / W, x % W, eo_u01(o[0]), &cr, &cc);
+      int child = cr * W + cc;
+      uint32_t key = eo_arbitration_key((uint32_t)i);
+      if (st[child] == EO_VACANT && key < best_key) {
+        best_key = key; best = i; best_child = child;
+      }
+    }
   }
-  int best = -1;
-  uint32_t best_key = 0xffffffffu;
-  int best_child = -1;
-  for (int i = 0; i < n; ++i) {
+  for (int i = 0; !q4 && i < n; ++i) {
     if (!(flags[i] & 0x20)) continue;
     int x = list[i];
     float u = eo_uniform(seed, epoch, sim, t, (uint32_t)x, EO_PURPOSE_NEIGHBOUR);
@@ -642,9 +680,10 @@ static int eo_run_one(const eo_params *p, float *sub, uint8_t *st, eo_cell *cell
                       uint16_t *list, int32_t *pn, int32_t cap, int32_t *roi,
                       uint32_t *pstep, uint32_t sim, const float *dose_row,
                       int32_t n_slots, uint32_t res, uint32_t length, uint32_t seed,
-                      uint32_t epoch, uint32_t nsteps, uint64_t *instr) {
+                      uint32_t epoch, uint32_t nsteps, uint64_t *instr, int rules) {
   const int H = p->H, W = p->W, HW = H * W;
   int32_t *owner = (int32_t *)malloc(sizeof(int32_t) * HW);
+  int32_t *seen = (int32_t *)calloc((size_t)HW, sizeof(int32_t));
   uint8_t *flags = (uint8_t *)malloc((size_t)cap + 1);
   int rc = 0;
   for (int i = 0; i < HW; ++i) owner[i] = -1;
@@ -660,14 +699,15 @@ static int eo_run_one(const eo_params *p, float *sub, uint8_t *st, eo_cell *cell
     eo_diffuse_f32(p, H, W, sub, roi);
     float dose = 0.f;
     if (t < length && t % res == 0 && (int32_t)(t / res) < n_slots) dose = dose_row[t / res];
-    if (eo_cells_step(p, H, W, sub, st, cells, list, pn, cap, owner, flags, t, sim, dose,
-                      seed, epoch, instr) != 0) {
+    if (eo_cells_step(p, H, W, sub, st, cells, list, pn, cap, owner, flags, seen, t, sim,
+                      dose, seed, epoch, instr, rules) != 0) {
       rc = -1;
       break;
     }
     *pstep = t + 1;
   }
   free(owner);
+  free(seen);
   free(flags);
   return rc;
 }
@@ -688,6 +728,17 @@ int eo_run_instr(const eo_params *p, int B, float *sub, uint8_t *st, eo_cell *ce
                  int32_t n_slots, uint32_t prot_res, uint32_t prot_length,
                  uint32_t seed, uint32_t epoch, uint32_t nsteps, int nthreads,
                  uint64_t *instr) {
+  return eo_run_rules(p, B, sub, st, cells, list, nlist, cap, roi, step, sim_id, prot_index,
+                      prot_data, n_slots, prot_res, prot_length, seed, epoch, nsteps, nthreads,
+                      instr, 0);
+}
+
+int eo_run_rules(const eo_params *p, int B, float *sub, uint8_t *st, eo_cell *cells,
+                 uint16_t *list, int32_t *nlist, int32_t cap, int32_t *roi, uint32_t *step,
+                 const int32_t *sim_id, const int32_t *prot_index, const float *prot_data,
+                 int32_t n_slots, uint32_t prot_res, uint32_t prot_length,
+                 uint32_t seed, uint32_t epoch, uint32_t nsteps, int nthreads,
+                 uint64_t *instr, int rules) {
   const size_t HW = (size_t)p->H * p->W;
   int err = 0;
 #ifdef _OPENMP
@@ -699,7 +750,7 @@ int eo_run_instr(const eo_params *p, int B, float *sub, uint8_t *st, eo_cell *ce
                         list + (size_t)b * cap, nlist + b, cap, roi + 5 * b, step + b,
                         (uint32_t)sim_id[b], prot_data + (size_t)prot_index[b] * n_slots,
                         n_slots, prot_res, prot_length, seed, epoch, nsteps,
-                        instr ? instr + (size_t)b * EO_INSTR_N : NULL);
+                        instr ? instr + (size_t)b * EO_INSTR_N : NULL, rules);
     if (rc != 0) err |= 1;
   }
   (void)nthreads;

@@ -136,6 +136,17 @@ int eo_run_instr(const eo_params *p, int B, float *sub, uint8_t *st, eo_cell *ce
                  uint32_t seed, uint32_t epoch, uint32_t nsteps, int nthreads,
                  uint64_t *instr);
 
+/* eo_run_instr with two of the deterministic step rules switched to the
+ * reference's literal behaviour (bit mask; 0 = the rules the product
+ * implements).  For A/B statistics of the rule choice only. */
+enum { EO_RULE_Q3_EVERY_ENTRY = 1, EO_RULE_Q4_LIST_ORDER = 2 };
+int eo_run_rules(const eo_params *p, int B, float *sub, uint8_t *st, eo_cell *cells,
+                 uint16_t *list, int32_t *nlist, int32_t cap, int32_t *roi, uint32_t *step,
+                 const int32_t *sim_id, const int32_t *prot_index, const float *prot_data,
+                 int32_t n_slots, uint32_t prot_res, uint32_t prot_length,
+                 uint32_t seed, uint32_t epoch, uint32_t nsteps, int nthreads,
+                 uint64_t *instr, int rules);
+
 uint32_t eo_count_living(int H, int W, const uint8_t *st);
 
 #ifdef __cplusplus

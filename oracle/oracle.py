@@ -16,6 +16,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libemt6ro_oracle.so")
 
 VACANT, OCCUPIED, MOCKED = 0, 1, 2
+# Step-rule switches of eo_run_rules (emt6ro_oracle.h): the reference's literal
+# duplicate-entry stepping and list-order vacancy reads, for A/B statistics.
+RULE_Q3_EVERY_ENTRY, RULE_Q4_LIST_ORDER = 1, 2
 # Instrumentation record (emt6ro_oracle.h, EO_INSTR_*): 5 counters, then two
 # 300-bin histograms.
 INSTR_N = 605
@@ -76,6 +79,8 @@ def _load():
     lib.eo_run.restype = C.c_int
     lib.eo_run_instr.argtypes = lib.eo_run.argtypes + [P]
     lib.eo_run_instr.restype = C.c_int
+    lib.eo_run_rules.argtypes = lib.eo_run_instr.argtypes + [C.c_int]
+    lib.eo_run_rules.restype = C.c_int
     lib.eo_count_living.argtypes = [C.c_int, C.c_int, P]
     lib.eo_count_living.restype = u32
     return lib
@@ -197,19 +202,19 @@ class OracleBatch:
         self.prot = np.ascontiguousarray(prot, np.float32)
         self.prot_res, self.prot_length = int(prot_res), int(prot_length)
 
-    def run(self, nsteps, seed, epoch=0, nthreads=1, instr=False):
+    def run(self, nsteps, seed, epoch=0, nthreads=1, instr=False, rules=0):
         """With instr=True, also accumulate the instrumentation counters of
         common/instr.h into self.instr (B x INSTR_N uint64, see
-        emt6ro_oracle.h)."""
+        emt6ro_oracle.h).  rules: RULE_* bits (0 = the product's rules)."""
         if instr and getattr(self, "instr", None) is None:
             self.instr = np.zeros((self.B, INSTR_N), np.uint64)
-        rc = lib.eo_run_instr(C.byref(self.p), self.B, ptr(self.sub), ptr(self.st),
+        rc = lib.eo_run_rules(C.byref(self.p), self.B, ptr(self.sub), ptr(self.st),
                               ptr(self.cells), ptr(self.list), ptr(self.nlist), self.cap,
                               ptr(self.roi), ptr(self.step), ptr(self.sim_id),
                               ptr(self.prot_index), ptr(self.prot), self.prot.shape[1],
                               self.prot_res, self.prot_length, int(seed), int(epoch),
                               int(nsteps), int(nthreads),
-                              ptr(self.instr) if instr else None)
+                              ptr(self.instr) if instr else None, int(rules))
         if rc != 0:
             raise RuntimeError("oracle: occupied list overflow")
 
