@@ -70,7 +70,10 @@ def assign_dose_within_time_constraint(rng, dose_steps, time_steps, interval_ste
 
 def refine_to_time_constraint(rng, genome, interval_in_indices, config):
     """Moves doses closer than the interval to random allowed slots
-    (refine_genome_around_cross_point_to_time_constraint)."""
+    (refine_genome_around_cross_point_to_time_constraint, utils.py:27-62).
+    The new slot is drawn from the genome's own slots; the reference draws
+    from 240 slots (the 120 h default of utils.py:9) whatever the genome's
+    length, which is the same for its 240-slot genomes."""
     genome = np.array(genome, dtype=float)
     nz = np.argwhere(genome > 0)[:, 0]
     gaps = np.diff(nz)
@@ -293,37 +296,42 @@ CROSSOVERS = {"cross_one_point": cross_one_point, "cross_two_points": cross_two_
 # ---------------------------------------------------------------------------
 def mutate_dose_value(rng, population, config):
     """Raises random genes to the largest dose the budget allows until the
-    budget is over 80 % used (the reference applies it to every genome; its
-    mut_prob is not read).  Bounded at 10,000 draws per genome."""
+    budget is over 80 % used (mutation.py:6-37; the reference applies it to
+    every genome and does not read mut_prob).  As there, the raises build up
+    on the genome as drawn, and the result is the time-constraint refinement
+    of its last state (each raise re-refines it).  Bounded at 10,000 draws
+    per genome."""
     interval = int(2 * config["time_interval_hours"])
     mc = config["mutations"]["mutate_dose_value"]
     cap_one, cap, min_dose = config["max_dose_value"], mc["max_value"], mc["min_value"]
     out = []
     for g in population:
         g = np.array(g, float)
+        result = g.copy()
         for _ in range(10000):
             idx = int(rng.integers(0, len(g)))
             limit = min(cap_one, cap - g.sum()) + g[idx]
             limit = min(limit, cap_one)
             if limit > min_dose:
                 g[idx] = limit
-                g = refine_to_time_constraint(rng, g, interval, config)
+                result = refine_to_time_constraint(rng, g, interval, config)
             elif limit == 0 or g.sum() > cap * 0.8:
                 break
-        out.append(g)
+        out.append(np.asarray(result, float))
     return out
 
 
 def mutate_time_value(rng, population, config):
+    """mutation.py:40-68: per gene, one uniform; below mut_prob a non-zero
+    dose moves to a random slot (capped at max_dose_value, the rest stays)."""
     interval = int(2 * config["time_interval_hours"])
     prob = config["mutations"]["mutate_time_value"]["mut_prob"]
     cap_one = config["max_dose_value"]
     out = []
     for g in population:
         g = np.array(g, float)
-        # One uniform per gene, drawn up front; genes are then visited in order.
-        for idx in np.nonzero(rng.random(len(g)) < prob)[0]:
-            if g[idx] > 0:
+        for idx in range(len(g)):
+            if rng.uniform() < prob and g[idx] > 0:
                 t = int(rng.integers(len(g)))
                 v = g[t] + g[idx]
                 rem = 0.0
@@ -336,12 +344,16 @@ def mutate_time_value(rng, population, config):
 
 
 def mutate_swap(rng, population, config):
+    """mutation.py:71-91: per gene, one uniform; below mut_prob the gene
+    swaps with another one at a random offset."""
     interval = int(2 * config["time_interval_hours"])
     prob = config["mutations"]["mutate_swap"]["mut_prob"]
     out = []
     for g in population:
         g = np.array(g, float)
-        for idx in np.nonzero(rng.random(len(g)) < prob)[0]:
+        for idx in range(len(g)):
+            if rng.uniform() >= prob:
+                continue
             k = (idx + int(rng.integers(len(g) - 1))) % len(g)
             g[idx], g[k] = g[k], g[idx]
         out.append(refine_to_time_constraint(rng, g, interval, config))

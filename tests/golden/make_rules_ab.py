@@ -50,19 +50,25 @@ def main():
     out["workload"] = {k: c[k] for k in ("size", "radius", "tumor_seed", "doses", "steps",
                                          "key_seed", "resolution")}
     for rules in rule_sets:
-        b = O.OracleBatch(values, [state] * n, doses, [0] * n, np.arange(n), c["resolution"],
-                          c["steps"])
-        t0 = time.time()
-        b.run(c["steps"], seed=c["key_seed"], nthreads=threads, rules=rules)
-        counts = b.counts().tolist()
-        out[f"rules_{rules}"] = {"counts": counts, "mean": float(np.mean(counts)),
-                                 "sd": float(np.std(counts, ddof=1)),
-                                 "seconds": time.time() - t0, "threads": threads}
-        print(json.dumps({"rules": rules, "mean": out[f"rules_{rules}"]["mean"],
-                          "sd": out[f"rules_{rules}"]["sd"],
-                          "seconds": out[f"rules_{rules}"]["seconds"]}), flush=True)
-        with open(path, "w") as f:
-            json.dump(out, f)
+        key = f"rules_{rules}"
+        rec = out.get(key, {"counts": [], "seconds": 0.0, "threads": threads})
+        # chunks of 100 replicates, saved as they finish (a rerun resumes)
+        while len(rec["counts"]) < n:
+            lo = len(rec["counts"])
+            hi = min(n, lo + 100)
+            b = O.OracleBatch(values, [state] * (hi - lo), doses, [0] * (hi - lo),
+                              np.arange(lo, hi), c["resolution"], c["steps"])
+            t0 = time.time()
+            b.run(c["steps"], seed=c["key_seed"], nthreads=threads, rules=rules)
+            rec["counts"] += b.counts().tolist()
+            rec["seconds"] += time.time() - t0
+            rec["mean"] = float(np.mean(rec["counts"]))
+            rec["sd"] = float(np.std(rec["counts"], ddof=1))
+            out[key] = rec
+            with open(path, "w") as f:
+                json.dump(out, f)
+            print(json.dumps({"rules": rules, "n": len(rec["counts"]), "mean": rec["mean"],
+                              "sd": rec["sd"], "seconds": rec["seconds"]}), flush=True)
 
 
 if __name__ == "__main__":
