@@ -5,17 +5,22 @@ Workload (batched uniform): 10 seeded synthetic 53x53 tumours (occupied disks
 of radius 8-14) x 100 random dose protocols x 20 replicates = 20,000
 simulations, 144 simulated hours (86,400 steps of 6 s), FP32 substrates.
 
-One bench step advances every simulation by one simulated hour (600 model
-steps): one launch of the window kernel (the simulations whose ROI fits a
-window of the lattice in LDS) and one of the whole-lattice persistent kernel
-(the rest); --steps 144 (the default) is the whole 144 h workload.  Warm-up steps run on the same batch, which is then
-reset to its initial states (untimed) before the timed steps.
+The timed region always covers the whole 144 h horizon: one bench step is
+144 / --steps simulated hours of every simulation, run as one-hour launches
+(the window kernel for the simulations whose ROI fits a window of the
+lattice in LDS, then the whole-lattice persistent kernel for the rest).
+Warm-up steps (the same size) run on the same batch, which is then reset to
+its initial states (untimed) before the timed steps.
 
-Also reported, on bounded samples:
-  * the per-step-launch path (the reference's structure: one launch per
-    phase per model step) on the same 20,000 simulations for 1 h;
-  * the CPU oracle on this host's cores for 16 of the simulations over the
-    full 144 h (cpu_baseline).
+Also reported:
+  * the LDS read bandwidth of the GPU (a probe kernel) and the simulator's
+    LDS traffic against it;
+  * the per-step-launch path (the reference's structure: per model step one
+    ROI, one diffusion and one cell launch, replayed from a HIP graph) on
+    every 12th simulated hour of the same run, against the persistent
+    path's time for the same hours;
+  * the CPU oracle on this host's cores and on one core (cpu_baseline);
+  * the fitness surrogate's training and screening rates.
 """
 import argparse
 import json
@@ -29,23 +34,32 @@ sys.path.insert(0, os.path.join(ROOT, "emt6-ro_amd"))
 METRIC = "simulated tumour-hours/sec on 1 MI355X; GA evals/sec surrogate vs plain"
 FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 peak
 FLOP_PER_SITE_SUBSTEP = 36    # 3 substrates x (6 adds + 3 fma x 2)
+# 12 B read of the three centre values + 12 B write of the new ones per
+# site-substep, neighbour reuse assumed perfect.
+LDS_BYTES_PER_SITE_SUBSTEP = 24
 FP32_NONPACKED_TFLOPS = 78.6  # the same without packed FP32 (the kernels are compiled without it)
-# HBM bytes per simulated hour of this workload (20,000 simulations): the
-# window kernel's and the persistent kernel's (variant 32) rocprofv3
-# FETCH_SIZE + WRITE_SIZE over a 2 + 144 h run, divided by its 146 hours, each
-# counter scaled by what it reads for a 1 GiB device copy (x2.0, x1.0).
-TRAFFIC_BYTES_PER_LAUNCH = 14.22e9 + 20.82e9
+HOURS = 144
+HOUR = 600                    # model steps per simulated hour
+# HBM bytes per simulated hour of this workload (both kernels of an hour),
+# from rocprofv3 FETCH_SIZE and WRITE_SIZE over a whole run, each scaled by
+# what the counter reads for a 1 GiB device copy; the profile below gives the
+# commands and figures.
+TRAFFIC_PROFILE = "profiles/persistent_kernel_batched_uniform_53x53.md"
+TRAFFIC_BYTES_PER_HOUR = 14.22e9 + 20.82e9
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=144, help="timed simulated hours")
-    ap.add_argument("--warmup", type=int, default=2, help="untimed simulated hours")
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed steps; each is 144/steps simulated hours, so the timed "
+                         "region is always the whole 144 h")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed steps (same size)")
     ap.add_argument("--variant", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the per-step path sample")
-    ap.add_argument("--cpu-sims", type=int, default=16)
+    ap.add_argument("--plain-every", type=int, default=12,
+                    help="time the per-step path on every N-th simulated hour")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner")
     ap.add_argument("--prefer", type=int, default=-1,
                     help="whole-lattice variant to use without tuning (the window pass stays on)")
@@ -64,16 +78,30 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(params, wl, n_sims, hours):
-    """The CPU oracle (test infrastructure) on this host's cores, on the first
-    n_sims simulations of the workload."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(params, wl, hours):
+    """The CPU oracle (test infrastructure) on this host: the first `cores`
+    simulations of the workload on `cores` OpenMP threads, one simulation per
+    thread, and simulation `cores` alone on one thread, each over the full
+    horizon.  cores = the CPUs this process may use, at most 16 (the GPU
+    box's share of its host)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
     v = params.values()
-    states = []
     cache = {}
-    for i in range(n_sims):
+
+    def state(i):
         t = int(wl["tumor_index"][i])
         if t not in cache:
             st = wl["tumors"][t]
@@ -92,17 +120,30 @@ def cpu_baseline(params, wl, n_sims, hours):
             cells["mode"][occ] = st.mode()[occ]
             cells["phase"][occ] = st.phase()[occ]
             cache[t] = s
-        states.append(cache[t])
-    cores = max(1, min(16, os.cpu_count() or 1, n_sims))
-    orc = O.OracleBatch(v, states, wl["doses"], wl["prot_index"][:n_sims],
-                        np.arange(n_sims), 300, wl["prot_length"])
-    t0 = time.perf_counter()
-    orc.run(hours * 600, seed=0, nthreads=cores)
-    dt = time.perf_counter() - t0
-    return {"value": n_sims * hours / dt, "unit": "tumour-hours/s", "cores": cores,
+        return cache[t]
+
+    def timed(idx, threads):
+        orc = O.OracleBatch(v, [state(i) for i in idx], wl["doses"], wl["prot_index"][idx],
+                            np.asarray(idx), 300, wl["prot_length"])
+        t0 = time.perf_counter()
+        orc.run(hours * HOUR, seed=0, nthreads=threads)
+        return time.perf_counter() - t0
+
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    cores = max(1, min(16, avail))
+    dt = timed(list(range(cores)), cores)
+    dt1 = timed([cores], 1)
+    return {"value": cores * hours / dt, "unit": "tumour-hours/s", "cores": cores,
             "kind": "port", "seconds": dt,
-            "sample": f"{n_sims} of the 20000 simulations (first in batch order) over the "
-                      f"full {hours} h, CPU oracle (C, FP32, OpenMP over simulations)"}
+            "sample": f"simulations 0-{cores - 1} of the 20000 (batch order), one per OpenMP "
+                      f"thread, over the full {hours} h: CPU oracle (C, FP32)",
+            "single_core": {"value": hours / dt1, "unit": "tumour-hours/s", "cores": 1,
+                            "seconds": dt1,
+                            "sample": f"simulation {cores} over the full {hours} h, 1 thread"},
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "cpus_available": avail}
 
 
 def surrogate_sample(train_steps=200, batch=4096, n_data=200000, n_screen=100000):
@@ -150,6 +191,25 @@ def copy_calibration(gib=1):
     print(json.dumps({"copy_bytes_each_way": n * 4, "copies": 3}), flush=True)
 
 
+def step_bounds(steps):
+    """Model-step boundaries of the timed steps: step i covers
+    [b[i], b[i+1]) of the 86,400-step horizon."""
+    total = HOURS * HOUR
+    return [total * i // steps for i in range(steps + 1)]
+
+
+def run_span(b, lo, hi, variant, hour_ms=None, path="persistent"):
+    """Advance every simulation from model step lo to hi in launches that end
+    on hour boundaries; per-hour device time (HIP events) goes to hour_ms."""
+    t = lo
+    while t < hi:
+        e = min(hi, (t // HOUR + 1) * HOUR)
+        ms = b.run(e - t, path=path, variant=variant)
+        if hour_ms is not None:
+            hour_ms[t // HOUR] = hour_ms.get(t // HOUR, 0.0) + ms
+        t = e
+
+
 def main():
     args = parse()
     if args.copy_calibrate:
@@ -162,11 +222,15 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the simulations are "
               "independent, so N GPUs would each run 1/N of the batch.", file=sys.stderr)
         sys.exit(2)
+    if args.steps < 1 or args.warmup < 0:
+        sys.exit("--steps must be >= 1 and --warmup >= 0")
     import numpy as np
     import emt6ro.simulation as sim
-    from emt6ro.batch import Batch, variants
+    from emt6ro._native import probe_lds
+    from emt6ro.batch import Batch
     from emt6ro.workloads import batched_uniform
 
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     params = sim.load_parameters(os.path.join(ROOT, "tests", "golden", "default-parameters.json"))
     wl = batched_uniform(params, seed=0)
     if args.sims:
@@ -175,31 +239,38 @@ def main():
     n = len(wl["tumor_index"])
     b = Batch(params, wl["tumors"], wl["tumor_index"], wl["prot_index"], wl["horizon"],
               wl["doses"], 300, wl["prot_length"], sim_id=np.arange(n), seed=0)
-    hour = 600
     tuned = None
     if args.prefer >= 0:
         b.set_variant(56, args.prefer)
     if args.variant < 0 and not args.no_tune:
         from emt6ro.tune import autotune
-        tuned = autotune(b, {56: (53, 53)}, steps=args.tune_steps, use_cache=False,
-                         log=lambda m: print(m, file=sys.stderr, flush=True),
+        tuned = autotune(b, {56: (53, 53)}, steps=args.tune_steps, use_cache=False, log=log,
                          refine_warm=args.tune_refine_warm)
-    for _ in range(args.warmup):
-        b.run(hour, path="persistent", variant=args.variant)
+    bounds = step_bounds(args.steps)
+    for i in range(args.warmup):
+        k = i % args.steps
+        run_span(b, bounds[k], bounds[k + 1], args.variant)
+        if k == args.steps - 1:
+            b.reset(0)
     b.reset(0)
-    # Timed region: every simulation advances args.steps hours.
-    event_ms = 0.0
+    log(f"[bench] warm-up done; timing {args.steps} steps of "
+        f"{HOURS / args.steps:g} h ({bounds[1] - bounds[0]} model steps)")
+    # Timed region: the whole 144 h of every simulation.
+    hour_ms = {}
     before = b.counters()["site_substeps"].astype(np.float64).sum()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        event_ms += b.run(hour, path="persistent", variant=args.variant)
+    for i in range(args.steps):
+        run_span(b, bounds[i], bounds[i + 1], args.variant, hour_ms)
     wall = time.perf_counter() - t0
     after = b.counters()["site_substeps"].astype(np.float64).sum()
     living = b.results()
-    value = n * args.steps / wall
-    flops = (after - before) * FLOP_PER_SITE_SUBSTEP
-    launch_s = event_ms / 1e3 / max(args.steps, 1)
-    achieved = flops / max(args.steps, 1) / launch_s / 1e12
+    log(f"[bench] timed {wall:.2f} s")
+    event_s = sum(hour_ms.values()) / 1e3
+    value = n * HOURS / wall
+    sub = after - before
+    achieved = sub * FLOP_PER_SITE_SUBSTEP / event_s / 1e12
+    probe = {w: probe_lds(w) for w in (16, 4)}
+    lds_gbps = sub * LDS_BYTES_PER_SITE_SUBSTEP / event_s / 1e9
     out = {
         "metric": METRIC,
         "value": value,
@@ -207,7 +278,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / max(args.steps, 1),
+        "ms_per_step": wall * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -217,50 +288,76 @@ def main():
             "workload": "batched uniform: 10 synthetic 53x53 tumours (disk radius 8-14) x 100 "
                         "random protocols (<=10 doses of 0.25-1.25 Gy, <=10 Gy, 0.5 h grid, "
                         "first 72 h) x 20 replicates = 20000 simulations, 144 h horizon",
-            "sims": n, "lattice": "53x53", "hours_timed": args.steps, "path": "persistent",
+            "sims": n, "lattice": "53x53", "hours_timed": HOURS,
+            "hours_per_step": HOURS / args.steps, "path": "persistent",
             "variant": args.variant, "mean_living": float(living.mean()),
             "tuned": tuned,
         },
         "roofline": {
-            "bound": "mfma", "pipe": "valu_fp32",
+            "bound": "valu", "pipe": "FP32 VALU (the diffusion stencil on LDS-resident "
+                                     "substrates)",
             "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP32_PEAK_TFLOPS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": TRAFFIC_BYTES_PER_HOUR * HOURS / args.steps,
+            "traffic_source": f"{TRAFFIC_PROFILE}: rocprofv3 FETCH_SIZE + WRITE_SIZE "
+                              "(copy-calibrated) per simulated hour x hours per step",
             "frac_of_nonpacked_fp32": achieved / FP32_NONPACKED_TFLOPS,
-            "hbm_gbps": TRAFFIC_BYTES_PER_LAUNCH / launch_s / 1e9,
-            "kernel": "window_kernel + persistent_kernel (one hour of every simulation)",
-            "launch_ms": launch_s * 1e3,
-            "site_substeps_per_launch": (after - before) / max(args.steps, 1),
-            "flop_per_site_substep": FLOP_PER_SITE_SUBSTEP,
+            "kernel": "window_kernel + persistent_kernel, one-hour launches",
+            "launch_ms_per_hour": sum(hour_ms.values()) / len(hour_ms),
+            "site_substeps": sub, "flop_per_site_substep": FLOP_PER_SITE_SUBSTEP,
+            "lds": {"achieved_gbps": lds_gbps,
+                    "bytes_per_site_substep": LDS_BYTES_PER_SITE_SUBSTEP,
+                    "peak_gbps_b128": probe[16]["bytes_per_s"] / 1e9,
+                    "peak_gbps_b32": probe[4]["bytes_per_s"] / 1e9,
+                    "peak_bytes_per_clk_cu_b128": probe[16]["bytes_per_clk_cu"],
+                    "peak_bytes_per_clk_cu_b32": probe[4]["bytes_per_clk_cu"],
+                    "probe_clock_mhz": probe[16]["clock_hz"] / 1e6,
+                    "frac": lds_gbps * 1e9 / probe[16]["bytes_per_s"]},
         },
     }
+    hours_sorted = sorted(hour_ms)
+    out["persistent_hour_ms"] = {"first": hour_ms[hours_sorted[0]],
+                                 "last": hour_ms[hours_sorted[-1]]}
     if not args.no_plain:
-        # Capture and instantiate the per-step graph outside the timed region.
+        # The per-step path on every N-th hour of the same run (the other hours
+        # advance on the persistent path, untimed); both paths give identical
+        # states, so the persistent path's times for the same hours compare
+        # like with like.  The 128-step graph is captured beforehand.
         b.reset(0)
         b.run(128, path="graph", variant=args.variant)
         b.reset(0)
-        t0 = time.perf_counter()
-        b.run(hour, path="graph", variant=args.variant)
-        plain = n / (time.perf_counter() - t0)
+        plain_ms, sampled = {}, []
+        for h in range(HOURS):
+            if h % args.plain_every == 0:
+                run_span(b, h * HOUR, (h + 1) * HOUR, args.variant, plain_ms, path="graph")
+                sampled.append(h)
+            else:
+                run_span(b, h * HOUR, (h + 1) * HOUR, args.variant)
+        same = bool(np.array_equal(b.results(), living))
+        p_ms = sum(plain_ms[h] for h in sampled)
+        q_ms = sum(hour_ms[h] for h in sampled)
         out["plain_per_step_launch"] = {
-            "value": plain, "unit": "tumour-hours/s",
-            "sample": "the same 20000 simulations, first hour (600 steps x 3 launches, "
-                      "replayed from a captured HIP graph)"}
-        b.reset(0)
-        t0 = time.perf_counter()
-        b.run(hour, path="persistent", variant=args.variant)
-        pers1 = n / (time.perf_counter() - t0)
-        out["persistent_first_hour"] = {"value": pers1, "unit": "tumour-hours/s"}
-        out["speedup_vs_plain_first_hour"] = pers1 / plain
+            "value": n * len(sampled) / (p_ms / 1e3), "unit": "tumour-hours/s",
+            "sample": f"hours {sampled[0]}, {sampled[1]}, ..., {sampled[-1]} of the same run "
+                      "(600 steps x 3 launches each, replayed from a captured HIP graph; "
+                      "device time)",
+            "persistent_same_hours": n * len(sampled) / (q_ms / 1e3),
+            "speedup": p_ms / q_ms, "final_counts_identical": same}
+        log(f"[bench] per-step path sample: {p_ms / len(sampled):.0f} ms/h vs "
+            f"{q_ms / len(sampled):.0f} ms/h persistent")
     if not args.no_surrogate:
         sur = surrogate_sample()
         out["surrogate"] = sur
         # One GA fitness evaluation = 3 tumours x 20 replicates x 144 h.
-        plain_evals = value / (3 * 20 * 144)
-        out["ga_evals_per_s"] = {"plain_simulator": plain_evals,
-                                 "surrogate_screen": sur["screen_rows_per_s"] / 3,
-                                 "ratio": sur["screen_rows_per_s"] / 3 / plain_evals}
+        plain_evals = value / (3 * 20 * HOURS)
+        out["ga_evals_per_s"] = {
+            "plain_simulator": plain_evals, "surrogate_screen": sur["screen_rows_per_s"] / 3,
+            "note": "an evaluation is 3 tumours x 20 replicates x 144 h on the simulator, or 3 "
+                    "surrogate rows; the GA's time to the plain GA's best fitness is measured "
+                    "by emt6ro.ga.experiment compare"}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(params, wl, args.cpu_sims, 144)
+        log("[bench] CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(params, wl, HOURS)
     line = json.dumps(out)
     print(line, flush=True)
     if args.json_out:

@@ -85,7 +85,7 @@ struct SimHdr {
   int32_t cur;              // slot buffer holding the list (0 or 1)
   int32_t roi[5];           // r0, c0, h, w, dist (h == 0: no diffusion)
   uint32_t divisions;       // divisions performed
-  uint32_t status;          // 0 ok; bit 0: slot overflow
+  uint32_t status;          // 0 ok; bit 0: slot overflow; bit 1: no diffusion layout fit
   uint64_t site_substeps;   // diffusion site-updates performed (roofline count)
   uint32_t pad[4];
 };

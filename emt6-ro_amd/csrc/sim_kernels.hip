@@ -150,11 +150,15 @@ __global__ __launch_bounds__(NT, WPE) void persistent_kernel(RunArgs a) {
 // the simulation stops before that step and a full-lattice launch takes it
 // to its target.
 // ---------------------------------------------------------------------------
-template <int RS>
+// A window takes an ROI whose temporary grid (h + 4 rows, w + 4 columns)
+// fits the window and whose mask box (h + 2 rows) is at most BOX rows, the
+// tallest the diffusion layout's lane slots hold (they are sized for a
+// lattice of row stride RS, however many rows the window has).
+template <int RS, int BOX>
 __device__ __forceinline__ bool window_origin(const int roi[5], int NR, int &wr, int &wc) {
   const int h = roi[2], w = roi[3];
   if (h == 0 || w == 0) { wr = 0; wc = 0; return true; }
-  if (h + 4 > NR || w + 4 > RS) return false;
+  if (h + 4 > NR || h + 2 > BOX || w + 4 > RS) return false;
   wr = roi[0] - 1 - (NR - (h + 4)) / 2;
   wc = roi[1] - 1 - (RS - (w + 4)) / 2;
   return true;
@@ -233,7 +237,7 @@ __global__ __launch_bounds__(NT, WPE) void window_kernel(RunArgs a) {
 #pragma unroll
     for (int q = 0; q < 5; ++q) roi[q] = hd->roi[q];
     int wr = 0, wc = 0;
-    if (nsteps == 0u || !window_origin<RS>(roi, NR, wr, wc)) {
+    if (nsteps == 0u || !window_origin<RS, Diff::kMaxBoxRows>(roi, NR, wr, wc)) {
       __syncthreads();
       continue;
     }
@@ -277,7 +281,7 @@ __global__ __launch_bounds__(NT, WPE) void window_kernel(RunArgs a) {
         if (!same) {
           if (!window_holds<RS>(nr, NR, wr, wc)) {
             int nwr, nwc;
-            if (!window_origin<RS>(nr, NR, nwr, nwc)) break;
+            if (!window_origin<RS, Diff::kMaxBoxRows>(nr, NR, nwr, nwc)) break;
             store_window<NT, RS>(a, d, win, PV, wr, wc);
             // The new window is loaded from what the old one just stored:
             // the stores must be complete before any thread's loads.
@@ -559,7 +563,7 @@ __global__ void set_target_kernel(const int32_t *sims, int n, const SimHdr *hdr,
 // side, longest first; the count goes to *count.
 __global__ __launch_bounds__(kOrderThreads) void order_window_kernel(
     const int32_t *src, int32_t *dst, int n, const SimHdr *hdr, const SimDesc *desc,
-    const uint32_t *target, int nr, int rs, int32_t *count) {
+    const uint32_t *target, int nr, int rs, int box, int32_t *count) {
   __shared__ int32_t cnt[kOrderBuckets];
   __shared__ int32_t wsum[kOrderThreads / 64];
   const int tid = threadIdx.x;
@@ -567,8 +571,9 @@ __global__ __launch_bounds__(kOrderThreads) void order_window_kernel(
     const SimHdr &h = hdr[sim];
     if (h.step >= target[sim]) return -1;
     const int hh = h.roi[2], ww = h.roi[3];
-    // One site of room on each side for the ROI to grow during the launch.
-    if (hh != 0 && ww != 0 && (hh + 6 > nr || ww + 6 > rs)) return -1;
+    // One site of room on each side for the ROI to grow during the launch
+    // (within the mask-box rows the window kernel's layout holds).
+    if (hh != 0 && ww != 0 && (hh + 6 > nr || hh + 4 > box || ww + 6 > rs)) return -1;
     return order_bucket(h, desc[sim]);
   };
   cnt[tid] = 0;
@@ -807,6 +812,12 @@ hipError_t launch_order_by_cost(const int32_t *src, int32_t *dst, int n, const S
 // CU ran slower than the 53x53 whole-lattice kernel.)
 using WindowDiff36 = dev::LaneRunDiffuser<128, 36, 3, 2, 4, 6, 8, 1>;
 using WindowDiff56 = dev::LaneRunDiffuser<256, 56, 2, 2, 4, 6, 8, 1>;
+// The window kernels take ROIs up to RS - 4 wide (window_origin) and at
+// most kMaxBoxRows - 2 tall: their two rings must fit the ring slots.
+static_assert(WindowDiff36::kRingSites >= 4 * (36 - 4 + WindowDiff36::kMaxBoxRows - 2) + 16,
+              "ring slots of the 36-column window");
+static_assert(WindowDiff56::kRingSites >= 4 * (56 - 4 + WindowDiff56::kMaxBoxRows - 2) + 16,
+              "ring slots of the 56-column window");
 #define EMT6RO_WINDOW_KERNEL_36 dev::window_kernel<128, 36, WindowDiff36, 4>
 #define EMT6RO_WINDOW_KERNEL_56 dev::window_kernel<256, 56, WindowDiff56, 4>
 
@@ -847,8 +858,9 @@ hipError_t launch_order_window(const int32_t *src, int32_t *dst, int n, const Si
                                const SimDesc *desc, const uint32_t *target, int rows, int rs,
                                int32_t *count, hipStream_t s) {
   if (n <= 0) return hipSuccess;
+  const int box = rs == 36 ? WindowDiff36::kMaxBoxRows : WindowDiff56::kMaxBoxRows;
   hipLaunchKernelGGL(dev::order_window_kernel, dim3(1), dim3(dev::kOrderThreads), 0, s, src, dst,
-                     n, hdr, desc, target, rows, rs, count);
+                     n, hdr, desc, target, rows, rs, box, count);
   return hipGetLastError();
 }
 

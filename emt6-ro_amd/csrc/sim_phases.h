@@ -863,6 +863,10 @@ struct LaneRunDiffuser {
   static_assert(RS < kRedInts, "the chord table and TempGrid's flag share sm.red");
   static constexpr int kPlanes = 3;
   static constexpr int KM = LR::kSlotsUpTo(H3);
+  // Tallest mask box (rows) whose height-H3 layout has no more bands than
+  // the lane slots were sized for (RS - 2 rows).
+  static constexpr int kMaxBoxRows = H3 * ((RS - 2 + H3 - 1) / H3);
+  static constexpr int kRingSites = PS * NT;  // TempGrid: 4 (h + w) + 16 at most
   uint32_t jac[KM], fin[KM];
   int L = 4;
   TempGrid<NT, RS, PS> tg;   // kept between ROI changes
@@ -916,6 +920,9 @@ struct LaneRunDiffuser {
         if (best_cost < 0 || cost < best_cost) { best = Lc; best_cost = cost; }
       }
     }
+    // No height fits only for a mask box taller or wider than RS - 2, which
+    // the kernels' launch checks exclude; flag it rather than diffuse a part.
+    if (best_cost < 0 && threadIdx.x == 0) sm.status |= 2u;
     L = Lc = best;
 #pragma unroll
     for (int k = 0; k < KM; ++k) {

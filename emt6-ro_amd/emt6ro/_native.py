@@ -69,7 +69,7 @@ def _load():
         "emt6ro_variant_info": ([i32, C.POINTER(i32)], C.c_int),
         "emt6ro_variant_occupancy": ([i32, i32, i32], C.c_int),
         "emt6ro_batch_set_variant": ([P, i32, i32], C.c_int),
-    "emt6ro_batch_set_window": ([P, i32], C.c_int),
+        "emt6ro_batch_set_window": ([P, i32], C.c_int),
         "emt6ro_batch_set_doses": ([P, f32p], C.c_int),
         "emt6ro_instr_read": ([C.POINTER(u64), C.POINTER(C.c_double)], C.c_int),
         "emt6ro_instr_reset": ([], C.c_int),
@@ -86,6 +86,7 @@ def _load():
         "emt6ro_mlp_predict": ([P, f32p, i32, f32p], C.c_int),
         "emt6ro_mlp_grads": ([P, f32p, f32p, i32, f32p, f32p], C.c_int),
         "emt6ro_mlp_destroy": ([P], None),
+        "emt6ro_probe_lds": ([i32, i32, C.POINTER(C.c_double)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -103,6 +104,17 @@ def check(rc):
     if rc != 0:
         raise RuntimeError(lib.emt6ro_last_error().decode())
     return rc
+
+
+def probe_lds(width_bytes=16, device=0):
+    """LDS read bandwidth of the device (emt6ro_probe_lds): bytes/s, bytes
+    per shader clock per CU, shader clock (Hz), CUs."""
+    out = (C.c_double * 4)()
+    rc = lib.emt6ro_probe_lds(int(device), int(width_bytes), out)
+    if rc != 0:
+        raise RuntimeError(f"emt6ro_probe_lds failed ({rc})")
+    return {"bytes_per_s": out[0], "bytes_per_clk_cu": out[1], "clock_hz": out[2],
+            "cus": int(out[3])}
 
 
 def check_mlp(rc):

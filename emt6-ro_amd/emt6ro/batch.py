@@ -40,6 +40,7 @@ class Batch:
                                       int(gpu_id), int(seed), C.byref(h)))
         self._h = h
         self.n = n
+        self.window = -1
 
     def run(self, nsteps, path="persistent", variant=-1, steps_per_launch=0):
         """Advance every simulation nsteps (capped by its horizon); returns the
@@ -82,8 +83,9 @@ class Batch:
         plus two rings fits `rows` x 36 padded sites (and, on lattices wider
         than 54 sites with rows=-1, a 56-column window) run first in a kernel
         holding only that window in LDS.  -1: automatic, 0: off.  Results do
-        not depend on it."""
+        not depend on it.  The setting is kept in `self.window`."""
         check(lib.emt6ro_batch_set_window(self._h, int(rows)))
+        self.window = int(rows)
 
     def set_doses(self, doses):
         d = np.ascontiguousarray(doses, np.float32)

@@ -79,6 +79,9 @@ def main(argv=None):
                     help="simulations per batched run")
     ap.add_argument("--limit-protocols", type=int, default=None)
     a = ap.parse_args(argv)
+    if a.seed is None:
+        # drawn here so that the metadata records the seed actually used
+        a.seed = int.from_bytes(os.urandom(8), "little")
     meta_out = a.meta_out or a.out + ".meta.json"
     params = sim.load_parameters(a.params)
     tumors = [sim.load_state(p, params) for p in a.tumors]

@@ -47,8 +47,10 @@ def autotune(batch, classes, steps=150, use_cache=True, log=None, refine_warm=0,
     state."""
     cache = _load() if use_cache else {}
     # The variants are timed on the whole-lattice kernel alone: the window
-    # pass (which takes the simulations with small ROIs first) stays off
-    # until the end.
+    # pass (which takes the simulations with small ROIs first, so that the
+    # whole-lattice kernel would see none of them early in a run) stays off
+    # until the end, when the caller's setting is restored.
+    window = getattr(batch, "window", -1)
     if hasattr(batch, "set_window"):
         batch.set_window(0)
     out = {}
@@ -89,7 +91,7 @@ def autotune(batch, classes, steps=150, use_cache=True, log=None, refine_warm=0,
         cache[key] = out[rs]
     batch.reset(0)
     if hasattr(batch, "set_window"):
-        batch.set_window(-1)
+        batch.set_window(window)
     if use_cache:
         _save(cache)
     return out

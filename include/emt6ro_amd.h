@@ -234,6 +234,15 @@ int emt6ro_instr_histograms(uint64_t *dst, int32_t n);
 int emt6ro_instr_enable(int32_t on);
 int emt6ro_instr_enabled(void);
 
+/* ---- Measurement (no reference counterpart) ------------------------------ */
+
+/* LDS read bandwidth of `device`, the peak against which the simulator's LDS
+ * traffic is reported: every lane of 16 workgroups per CU reads 32 KB of
+ * LDS in lane-contiguous (conflict-free) order with loads of width_bytes (4
+ * or 16).  out[0] bytes/s over the GPU, out[1] bytes per shader clock per CU,
+ * out[2] the shader clock (Hz) under the probe, out[3] the CU count. */
+int emt6ro_probe_lds(int32_t device, int32_t width_bytes, double out[4]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,12 +95,31 @@ def test_paths_agree_through_experiment(sim, params, tumors):
     assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
 
 
-def test_ga_model_predict(sim, params, tumors):
+def test_ga_model_predict_is_1500_minus_mean_oracle_counts(sim, params, tumors):
+    """EMT6RoModel.predict (model.py:36-54): fitness = 1500 - mean final count
+    over tumours x tests of each protocol.  Its Experiment runs simulation
+    b = p*T*R + t*R + r on stream b of (seed, epoch); the oracle on the same
+    streams gives the counts, and the second call (after the model's reset)
+    draws epoch 1."""
+    import oracle as O
     from emt6ro.ga_model import EMT6RoModel
-    m = EMT6RoModel(params, tumors, num_protocols=2, num_tests=2, seed=1, num_steps=600)
-    f = m.predict([[(0, 1.0)], []])
-    assert f.shape == (2,)
-    assert np.all(f > 1000) and np.all(f < 1500)
+    log("[experiment] EMT6RoModel.predict")
+    P, T, R, steps, seed = 2, 2, 3, 600, 5
+    protocols = [[(0, 1.0), (300, 0.5), (300, 0.25)], []]
+    m = EMT6RoModel(params, tumors, num_protocols=P, num_tests=R, seed=seed, num_steps=steps)
+    v = O.read_params_json(golden("default-parameters.json"))
+    states = [O.read_state_text(golden("tumor-lib", f"tumor-{t}.txt"), v) for t in (1, 2)]
+    doses = np.zeros((P, 2), np.float32)
+    doses[0] = [1.0, np.float32(0.5) + np.float32(0.25)]
+    b = np.arange(P * T * R)
+    for epoch in (0, 1):
+        f = m.predict(protocols)
+        orc = O.OracleBatch(v, [states[(i // R) % T] for i in b], doses, b // (T * R), b, 300,
+                            steps)
+        orc.run(steps, seed=seed, epoch=epoch, nthreads=4)
+        counts = orc.counts().reshape(P, T * R)
+        assert f.shape == (P,)
+        assert np.array_equal(f, 1500 - counts.astype(np.float64).mean(axis=1)), (f, counts)
 
 
 def test_gpu_diffusion_within_1e5_of_fp64(sim, params):
