@@ -147,6 +147,25 @@ class Mlp {
     EMT_HIPM(hipStreamSynchronize(s_));
   }
 
+  // predict() on device rows, with no allocation: X_dev [n x in_] and
+  // y_dev [n] are device memory of this network's device.
+  void predict_noalloc(const float *X_dev, int n, float *y_dev) {
+    EMT_HIPM(hipSetDevice(device_));
+    EMT_ENFORCE(n >= 0, -1, "n must be >= 0; got ", n);
+    EMT_ENFORCE(in_ <= kNoAllocMaxCols && kin_ <= kNoAllocMaxCols && H_ <= kNoAllocMaxCols, -1,
+                "the allocation-free forward pass takes at most ", kNoAllocMaxCols,
+                " input and hidden features; this network has ", in_, " and ", H_);
+    if (n == 0) return;
+    EMT_ENFORCE(X_dev && y_dev, -1, "null device pointer");
+    MlpForward net{};
+    for (int l = 0; l < kLayers; ++l) { net.wb[l] = wb_[l]; net.bias[l] = w_ + offs_[2 * l + 1]; }
+    net.w4 = w_ + offs_[6];
+    net.b4 = w_ + offs_[7];
+    net.in = in_; net.kin = kin_; net.H = H_;
+    EMT_HIPM(mlp_forward_noalloc(net, device_, X_dev, n, y_dev, s_));
+    EMT_HIPM(hipStreamSynchronize(s_));
+  }
+
   // Loss and gradient (flat, unpadded) of one batch, without an update.
   float grads(const float *X, const float *t, int n, float *gout) {
     EMT_HIPM(hipSetDevice(device_));
@@ -370,6 +389,13 @@ int emt6ro_mlp_train(emt6ro_mlp *m, int32_t steps, int32_t batch, float lr, uint
 
 int emt6ro_mlp_predict(emt6ro_mlp *m, const float *X, int32_t n, float *y) {
   return mlp_guarded([&] { m->m->predict(X, n, y); });
+}
+
+int emt6ro_mlp_predict_noalloc(emt6ro_mlp *m, const float *X_dev, int32_t n, float *y_dev) {
+  return mlp_guarded([&] {
+    if (!m) throw emt6ro::Error("Check failed: m; null network", EMT6RO_ERR_ARG);
+    m->m->predict_noalloc(X_dev, n, y_dev);
+  });
 }
 
 int emt6ro_mlp_grads(emt6ro_mlp *m, const float *X, const float *y, int32_t n, float *grads,

@@ -47,6 +47,20 @@ hipError_t mlp_gather(const uint16_t *X, const float *t, uint64_t seed, uint32_t
                       uint16_t *Xb, float *tb, int B, int ld, hipStream_t s);
 hipError_t mlp_mean(const float *sq, int B, float *out, hipStream_t s);
 
+// Forward pass that allocates nothing: activations in a fixed device array
+// of three kNoAllocRows x kNoAllocMaxCols bf16 buffers (mlp_kernels.hip).
+constexpr int kNoAllocRows = 4096;      // rows per pass (the predict chunk)
+constexpr int kNoAllocMaxCols = 512;    // in, padded in and hidden at most
+struct MlpForward {
+  const uint16_t *wb[3];    // bf16 weights of the hidden layers
+  const float *bias[3];
+  const float *w4, *b4;     // output layer (fp32)
+  int in, kin, H;
+};
+// y[0..n) = network(X rows of `in` fp32 columns); X and y in device memory.
+hipError_t mlp_forward_noalloc(const MlpForward &net, int device, const float *X, int n, float *y,
+                               hipStream_t s);
+
 }  // namespace emt6ro
 
 #endif  // EMT6RO_AMD_MLP_H_

@@ -9,6 +9,7 @@
 // with v_mfma_f32_32x32x8_bf16 over K in steps of 32 staged through LDS.
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <mutex>
 
 #include "mlp.h"
 
@@ -461,6 +462,81 @@ hipError_t mlp_gather(const uint16_t *X, const float *t, uint64_t seed, uint32_t
 hipError_t mlp_mean(const float *sq, int B, float *out, hipStream_t s) {
   hipLaunchKernelGGL(mlp::mean_kernel, dim3(1), dim3(256), 0, s, sq, B, out);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Forward pass that allocates nothing (emt6ro_mlp_predict_noalloc): its
+// activations live in a fixed device array, three bf16 buffers of
+// kNoAllocRows x kNoAllocMaxCols (the padded input and two hidden layers,
+// used in turn), 12 MiB that each device holds until the process ends.
+// Every call on a device shares that device's copy, so calls that overlap on
+// one device race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) uint16_t scratch[3 * (size_t)kNoAllocRows * kNoAllocMaxCols];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint16_t *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+namespace mlp {
+// Rows of X (in columns, fp32) -> bf16 rows of kin columns, zero padded;
+// the same rounding as f32_to_bf16_kernel.
+__global__ void pad_to_bf16_kernel(const float *X, int64_t rows, int in, int kin, uint16_t *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * kin) return;
+  const int64_t r = i / kin;
+  const int c = (int)(i - r * kin);
+  out[i] = c < in ? to_bf16(X[r * in + c]) : (uint16_t)0;
+}
+}  // namespace mlp
+
+hipError_t mlp_noalloc_scratch(int device, uint16_t **out) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    // The array's device address on this device (the caller made it current);
+    // the host-side name `scratch` is not a device address.
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<uint16_t *>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
+hipError_t mlp_forward_noalloc(const MlpForward &net, int device, const float *X, int n, float *y,
+                               hipStream_t s) {
+  if (net.in > kNoAllocMaxCols || net.kin > kNoAllocMaxCols || net.H > kNoAllocMaxCols)
+    return hipErrorInvalidValue;
+  uint16_t *base = nullptr;
+  hipError_t e = mlp_noalloc_scratch(device, &base);
+  if (e != hipSuccess) return e;
+  const size_t buf = (size_t)kNoAllocRows * kNoAllocMaxCols;
+  uint16_t *xb = base, *h[2] = {base + buf, base + 2 * buf};
+  for (int r0 = 0; r0 < n; r0 += kNoAllocRows) {
+    const int b = n - r0 < kNoAllocRows ? n - r0 : kNoAllocRows;
+    hipLaunchKernelGGL(mlp::pad_to_bf16_kernel, dim3(blocks((int64_t)b * net.kin, 256)), dim3(256),
+                       0, s, X + (int64_t)r0 * net.in, (int64_t)b, net.in, net.kin, xb);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const uint16_t *in = xb;
+    int K = net.kin;
+    for (int l = 0; l < 3; ++l) {
+      MlpGemm q{};
+      q.A = in; q.sam = K; q.sak = 1;
+      q.B = net.wb[l]; q.sbk = 1; q.sbn = K;
+      q.M = b; q.N = net.H; q.K = K;
+      q.epi = 0; q.bias = net.bias[l]; q.act = 1;
+      q.z_out = nullptr; q.h_out = h[l & 1]; q.ldc = net.H;
+      if ((e = mlp_gemm(q, s)) != hipSuccess) return e;
+      in = h[l & 1];
+      K = net.H;
+    }
+    if ((e = mlp_gemv_out(in, net.w4, net.b4, y + r0, b, net.H, s)) != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace emt6ro

@@ -210,6 +210,17 @@ int emt6ro_mlp_set_data(emt6ro_mlp *m, const float *X, const float *y, int32_t n
 int emt6ro_mlp_train(emt6ro_mlp *m, int32_t steps, int32_t batch, float lr, uint64_t seed,
                      float *losses);
 int emt6ro_mlp_predict(emt6ro_mlp *m, const float *X, int32_t n, float *y);
+/* emt6ro_mlp_predict for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight screening loops, harnesses that forbid
+ * hipMalloc); no counterpart in the reference.  X_dev [n x in_features]
+ * and y_dev [n] are device memory of the network's device; it calls no
+ * allocator: activations live in a fixed 12 MiB device array per GPU (three
+ * bf16 buffers of 4096 rows x 512 columns), so in_features and hidden must be
+ * at most 512 (larger networks are rejected with an error); n is any size
+ * (4096 rows per pass).  The array is shared by every call on that GPU:
+ * calls that overlap on one GPU race.  Waits for the result, like
+ * emt6ro_mlp_predict, whose output it matches bit for bit. */
+int emt6ro_mlp_predict_noalloc(emt6ro_mlp *m, const float *X_dev, int32_t n, float *y_dev);
 /* Loss (mean squared error / 2) and its gradient for one batch, no update. */
 int emt6ro_mlp_grads(emt6ro_mlp *m, const float *X, const float *y, int32_t n, float *grads,
                      float *loss);
