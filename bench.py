@@ -45,7 +45,7 @@ HOUR = 600                    # model steps per simulated hour
 # what the counter reads for a 1 GiB device copy; the profile below gives the
 # commands and figures.
 TRAFFIC_PROFILE = "profiles/persistent_kernel_batched_uniform_53x53.md"
-TRAFFIC_BYTES_PER_HOUR = 14.22e9 + 20.82e9
+TRAFFIC_BYTES_PER_HOUR = 16.4e9 + 26.4e9
 
 
 def parse():
