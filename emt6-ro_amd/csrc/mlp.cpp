@@ -44,10 +44,14 @@ class Mlp {
     EMT_HIPM(hipMemsetAsync(v_, 0, tot * 4, s_));
     EMT_HIPM(hipMemsetAsync(g_, 0, tot * 4, s_));
     for (int l = 0; l < kLayers; ++l) wb_[l] = dalloc<uint16_t>(sizes_[2 * l]);
+    // Transposed bf16 copies of W2 and W3 ([in x out]), the k-contiguous
+    // operand of the input-gradient products.
+    for (int l = 1; l < kLayers; ++l) wbt_[l] = dalloc<uint16_t>(sizes_[2 * l]);
     // Split-K slices of the weight-gradient products: up to 16 slices of the
-    // largest weight matrix.
+    // largest weight matrix, for each hidden layer (summed together at the
+    // end of the backward pass).
     kwork_floats_ = 16 * std::max<int64_t>((int64_t)H_ * kin_, (int64_t)H_ * H_);
-    kwork_ = dalloc<float>(kwork_floats_);
+    kwork_ = dalloc<float>(kLayers * kwork_floats_);
     // PyTorch-style init: U(-1/sqrt(fan_in), 1/sqrt(fan_in)).
     std::mt19937_64 rng(seed);
     std::vector<float> h(tot, 0.f);
@@ -123,8 +127,9 @@ class Mlp {
     if (steps == 0) return losses;
     float *dl = dalloc<float>(steps);
     for (int it = 0; it < steps; ++it) {
-      EMT_HIPM(mlp_gather(data_x_, data_t_, seed, (uint32_t)it, n_data_, x_, t_, batch, kin_, s_));
-      forward(batch);
+      EMT_HIPM(mlp_gather(data_x_, data_t_, seed, (uint32_t)it, n_data_, x_, xt_, t_, batch, kin_,
+                          s_));
+      forward(batch, true);
       backward(batch);
       adam(lr);
       EMT_HIPM(mlp_mean(sq_, batch, dl + it, s_));
@@ -170,7 +175,8 @@ class Mlp {
   float grads(const float *X, const float *t, int n, float *gout) {
     EMT_HIPM(hipSetDevice(device_));
     load_batch(X, t, n);
-    forward(n);
+    EMT_HIPM(mlp_transpose_bf16(x_, n, kin_, xt_, s_));
+    forward(n, true);
     backward(n);
     std::vector<float> h(total_), sq(n);
     EMT_HIPM(hipMemcpyAsync(h.data(), g_, total_ * 4, hipMemcpyDeviceToHost, s_));
@@ -216,8 +222,10 @@ class Mlp {
 
   void upload_params(const std::vector<float> &h) {
     EMT_HIPM(hipMemcpyAsync(w_, h.data(), total_ * 4, hipMemcpyHostToDevice, s_));
-    for (int l = 0; l < kLayers; ++l)
+    for (int l = 0; l < kLayers; ++l) {
       EMT_HIPM(mlp_to_bf16(w_ + offs_[2 * l], wb_[l], sizes_[2 * l], s_));
+      if (wbt_[l]) EMT_HIPM(mlp_transpose_bf16(wb_[l], H_, H_, wbt_[l], s_));
+    }
     EMT_HIPM(hipStreamSynchronize(s_));
   }
 
@@ -227,9 +235,18 @@ class Mlp {
     release(x_);
     for (int l = 0; l < kLayers; ++l) { release(z_[l]); release(h_[l]); release(dz_[l]); }
     release(y_); release(t_); release(dy_); release(sq_); release(ones_); release(xs_);
-    release(cwork_);
+    release(cwork_); release(xt_);
+    for (int l = 0; l < kLayers; ++l) { release(ht_[l]); release(dzt_[l]); }
     bmax_ = B;
     x_ = dalloc<uint16_t>((size_t)B * kin_);
+    // Transposed copies ([features x batch]) of the layer inputs and of the
+    // pre-activation gradients: the k-contiguous operands of the
+    // weight-gradient products.
+    xt_ = dalloc<uint16_t>((size_t)B * kin_);
+    for (int l = 0; l < kLayers; ++l) {
+      if (l < kLayers - 1) ht_[l] = dalloc<uint16_t>((size_t)B * H_);
+      dzt_[l] = dalloc<uint16_t>((size_t)B * H_);
+    }
     xs_ = dalloc<float>((size_t)B * kin_);
     for (int l = 0; l < kLayers; ++l) {
       z_[l] = dalloc<uint16_t>((size_t)B * H_);
@@ -238,7 +255,7 @@ class Mlp {
     }
     y_ = dalloc<float>(B); t_ = dalloc<float>(B); dy_ = dalloc<float>(B); sq_ = dalloc<float>(B);
     ones_ = dalloc<uint16_t>(B);
-    cwork_ = dalloc<float>((size_t)((B + 127) / 128) * H_);
+    cwork_ = dalloc<float>((size_t)(2 + kLayers) * ((B + 127) / 128) * H_);
     std::vector<uint16_t> one(B, 0x3f80);  // bf16 1.0
     EMT_HIPM(hipMemcpyAsync(ones_, one.data(), B * 2, hipMemcpyHostToDevice, s_));
     EMT_HIPM(hipStreamSynchronize(s_));
@@ -256,7 +273,8 @@ class Mlp {
     EMT_HIPM(hipStreamSynchronize(s_));
   }
 
-  void forward(int B) {
+  // keep_t: also write the transposed layer outputs the backward pass needs.
+  void forward(int B, bool keep_t = false) {
     const uint16_t *in = x_;
     int K = kin_;
     for (int l = 0; l < kLayers; ++l) {
@@ -266,6 +284,7 @@ class Mlp {
       q.M = B; q.N = H_; q.K = K;
       q.epi = 0; q.bias = w_ + offs_[2 * l + 1]; q.act = 1;
       q.z_out = z_[l]; q.h_out = h_[l]; q.ldc = H_;
+      if (keep_t && l < kLayers - 1) { q.ht_out = ht_[l]; q.ldt = B; }
       EMT_HIPM(mlp_gemm(q, s_));
       in = h_[l];
       K = H_;
@@ -275,41 +294,62 @@ class Mlp {
   }
 
   void backward(int B) {
-    // Output layer: dw4 = dy^T h3, db4 = sum dy, dz3 = dy w4 gelu'(z3).
-    EMT_HIPM(mlp_colsum(h_[2], dy_, g_ + offs_[6], B, H_, cwork_, s_));
-    EMT_HIPM(mlp_colsum(ones_, dy_, g_ + offs_[7], B, 1, cwork_, s_));
-    EMT_HIPM(mlp_out_back(dy_, w_ + offs_[6], z_[2], dz_[2], B, H_, s_));
+    // Output layer: dz3 = dy w4 gelu'(z3); its gradients dw4 = dy^T h3 and
+    // db4 = sum dy are column sums, taken with the hidden biases' at the end.
+    EMT_HIPM(mlp_out_back(dy_, w_ + offs_[6], z_[2], dz_[2], dzt_[2], B, H_, s_));
+    SliceJobs slices{};
+    int nslices = 0;
     for (int l = kLayers - 1; l >= 0; --l) {
-      const uint16_t *in = l == 0 ? x_ : h_[l - 1];
+      const uint16_t *in_t = l == 0 ? xt_ : ht_[l - 1];
       const int K = l == 0 ? kin_ : H_;
-      // dW_l = dz_l^T in  ([H, K], reduction over the batch)
+      // dW_l = dz_l^T in  ([H, K], reduction over the batch), from the
+      // transposed copies so both operands are contiguous along the batch;
+      // split-K slices are summed for all layers in one launch below.
       MlpGemm q{};
-      q.A = dz_[l]; q.sam = 1; q.sak = H_;
-      q.B = in; q.sbk = K; q.sbn = 1;
+      q.A = dzt_[l]; q.sam = B; q.sak = 1;
+      q.B = in_t; q.sbk = 1; q.sbn = B;
       q.M = H_; q.N = K; q.K = B;
       q.epi = 2; q.f_out = g_ + offs_[2 * l]; q.ldc = K;
-      q.work = kwork_; q.work_floats = kwork_floats_;
+      q.work = kwork_ + l * kwork_floats_; q.work_floats = kwork_floats_;
+      int split = 1;
+      q.split_out = &split;
+      q.defer_sum = 1;
       EMT_HIPM(mlp_gemm(q, s_));
-      EMT_HIPM(mlp_colsum(dz_[l], nullptr, g_ + offs_[2 * l + 1], B, H_, cwork_, s_));
+      if (split > 1)
+        slices.job[nslices++] = SliceJob{q.work, (int64_t)H_ * K, split, q.f_out, (int64_t)H_ * K};
       if (l > 0) {
-        // dz_{l-1} = (dz_l W_l) * gelu'(z_{l-1})
+        // dz_{l-1} = (dz_l W_l) * gelu'(z_{l-1}), W_l from its transposed copy
         MlpGemm r{};
         r.A = dz_[l]; r.sam = H_; r.sak = 1;
-        r.B = wb_[l]; r.sbk = H_; r.sbn = 1;
+        r.B = wbt_[l]; r.sbk = 1; r.sbn = H_;
         r.M = B; r.N = H_; r.K = H_;
         r.epi = 1; r.z_in = z_[l - 1]; r.h_out = dz_[l - 1]; r.ldc = H_;
+        r.ht_out = dzt_[l - 1]; r.ldt = B;
         EMT_HIPM(mlp_gemm(r, s_));
       }
     }
+    if (nslices) EMT_HIPM(mlp_sum_slices_multi(slices, nslices, s_));
+    // Column sums: dw4, db4 and the three hidden biases.
+    const size_t cw = (size_t)((B + 127) / 128) * H_;
+    ColsumJobs cj{};
+    cj.job[0] = ColsumJob{h_[2], dy_, g_ + offs_[6], cwork_, H_};
+    cj.job[1] = ColsumJob{ones_, dy_, g_ + offs_[7], cwork_ + cw, 1};
+    for (int l = 0; l < kLayers; ++l)
+      cj.job[2 + l] = ColsumJob{dz_[l], nullptr, g_ + offs_[2 * l + 1], cwork_ + (2 + l) * cw, H_};
+    EMT_HIPM(mlp_colsum_multi(cj, 2 + kLayers, B, s_));
   }
 
   void adam(float lr) {
     ++step_;
-    for (int i = 0; i < 8; ++i) {
-      uint16_t *wb = (i % 2 == 0 && i < 6) ? wb_[i / 2] : nullptr;
-      EMT_HIPM(mlp_adam(w_ + offs_[i], m_ + offs_[i], v_ + offs_[i], g_ + offs_[i], wb,
-                        (int)sizes_[i], lr, 0.9f, 0.999f, 1e-8f, step_, s_));
+    AdamBf16 seg{};
+    for (int l = 0; l < kLayers; ++l) {
+      seg.wb[l] = wb_[l];
+      seg.wbt[l] = wbt_[l];
+      seg.off[l] = (int)offs_[2 * l];
+      seg.n[l] = (int)sizes_[2 * l];
+      seg.cols[l] = l == 0 ? kin_ : H_;
     }
+    EMT_HIPM(mlp_adam_all(w_, m_, v_, g_, seg, (int)total_, lr, 0.9f, 0.999f, 1e-8f, step_, s_));
   }
 
   int in_, kin_, H_, device_;
@@ -317,9 +357,10 @@ class Mlp {
   std::vector<int64_t> sizes_, offs_;
   int64_t total_ = 0;
   float *w_ = nullptr, *m_ = nullptr, *v_ = nullptr, *g_ = nullptr;
-  uint16_t *wb_[kLayers] = {};
+  uint16_t *wb_[kLayers] = {}, *wbt_[kLayers] = {};
   int bmax_ = 0;
   uint16_t *x_ = nullptr, *z_[kLayers] = {}, *h_[kLayers] = {}, *dz_[kLayers] = {}, *ones_ = nullptr;
+  uint16_t *xt_ = nullptr, *ht_[kLayers] = {}, *dzt_[kLayers] = {};
   float *y_ = nullptr, *t_ = nullptr, *dy_ = nullptr, *sq_ = nullptr, *xs_ = nullptr;
   float *cwork_ = nullptr, *kwork_ = nullptr;
   int64_t kwork_floats_ = 0;

@@ -8,6 +8,7 @@
 // tile of C; each of its four waves owns a 32x32 quarter and accumulates it
 // with v_mfma_f32_32x32x8_bf16 over K in steps of 32 staged through LDS.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
 #include <mutex>
 
@@ -28,14 +29,24 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
 }
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// tanh(u) = sign(u) (1 - e) / (1 + e), e = exp(-2 |u|), from the hardware
+// exp2 and reciprocal (absolute error a few 1e-8, far below the bf16
+// rounding of every value it feeds); the GEMM epilogues evaluate it once per
+// output element.
+__device__ __forceinline__ float fast_tanh(float u) {
+  const float e = __builtin_amdgcn_exp2f(-2.8853900817779268f * fabsf(u));  // exp(-2|u|)
+  const float t = (1.f - e) * __builtin_amdgcn_rcpf(1.f + e);
+  return copysignf(t, u);
+}
+
 // GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))).
 __device__ __forceinline__ float gelu(float x) {
   const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
-  return 0.5f * x * (1.f + tanhf(u));
+  return 0.5f * x * (1.f + fast_tanh(u));
 }
 __device__ __forceinline__ float gelu_grad(float x) {
   const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
-  const float t = tanhf(u);
+  const float t = fast_tanh(u);
   return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * 0.7978845608028654f * (1.f + 0.134145f * x * x);
 }
 
@@ -62,6 +73,8 @@ struct GemmArgs {
   int64_t ldc;
   int k_chunk;                 // reduction slice of one blockIdx.z (multiple of kTK)
   int64_t split_stride;        // kEpiStoreF32 with split K: floats between slices
+  uint16_t *ht_out;            // optional transposed copy of h_out: ht_out[n * ldt + m]
+  int64_t ldt;
 };
 
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
@@ -117,64 +130,81 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
       const float z = v + g.bias[n];
       const uint16_t zb = to_bf16(z);
       if (g.z_out) g.z_out[o] = zb;
-      g.h_out[o] = g.act ? to_bf16(gelu(from_bf16(zb))) : zb;
+      const uint16_t hb = g.act ? to_bf16(gelu(from_bf16(zb))) : zb;
+      g.h_out[o] = hb;
+      if (g.ht_out) g.ht_out[(int64_t)n * g.ldt + m] = hb;
     } else if (g.epi == kEpiBackGelu) {
-      g.h_out[o] = to_bf16(v * gelu_grad(from_bf16(g.z_in[o])));
+      const uint16_t hb = to_bf16(v * gelu_grad(from_bf16(g.z_in[o])));
+      g.h_out[o] = hb;
+      if (g.ht_out) g.ht_out[(int64_t)n * g.ldt + m] = hb;
     } else {
       g.f_out[(int64_t)blockIdx.z * g.split_stride + o] = v;
     }
   }
 }
 
-// Epilogue of one accumulator element (row m, column n, value v).
-__device__ __forceinline__ void gemm_store(const GemmArgs &g, int m, int n, float v) {
-  if (m >= g.M || n >= g.N) return;
+// Epilogue of one accumulator element (row m, column n, value v); returns
+// the bf16 value stored to h_out (epilogues 0 and 1).
+__device__ __forceinline__ uint16_t gemm_store(const GemmArgs &g, int m, int n, float v) {
+  if (m >= g.M || n >= g.N) return 0;
   const int64_t o = (int64_t)m * g.ldc + n;
   if (g.epi == kEpiForward) {
     const float z = v + g.bias[n];
     const uint16_t zb = to_bf16(z);
     if (g.z_out) g.z_out[o] = zb;
-    g.h_out[o] = g.act ? to_bf16(gelu(from_bf16(zb))) : zb;
+    const uint16_t hb = g.act ? to_bf16(gelu(from_bf16(zb))) : zb;
+    g.h_out[o] = hb;
+    return hb;
   } else if (g.epi == kEpiBackGelu) {
-    g.h_out[o] = to_bf16(v * gelu_grad(from_bf16(g.z_in[o])));
+    const uint16_t hb = to_bf16(v * gelu_grad(from_bf16(g.z_in[o])));
+    g.h_out[o] = hb;
+    return hb;
   } else {
     g.f_out[(int64_t)blockIdx.z * g.split_stride + o] = v;
   }
+  return 0;
 }
 
-// The same product on 128 x 128 tiles with 16-byte loads, for M, N, K and
-// the strides multiples of 8.  Each of the four waves owns a 64 x 64 quarter
-// as 2 x 2 blocks of v_mfma_f32_32x32x16_bf16 (lane l holds row l % 32 and
-// the 8 consecutive k from 8 (l / 32) of each operand).  AK / BK: operand A
-// (resp. B) is contiguous along k; otherwise along m (resp. n), and its
-// 8-element chunks are transposed into LDS.  The next k-tile is fetched into
-// registers while the current one is multiplied.
+// The same product on TM x TN tiles (128 x 128, 128 x 64 or 64 x 64) with 16-byte
+// loads, for M, N, K and the strides multiples of 8.  The four waves split
+// the tile 2 x 2; each owns (TM / 2) x (TN / 2) as blocks of
+// v_mfma_f32_32x32x16_bf16 (lane l holds row l % 32 and the 8 consecutive k
+// from 8 (l / 32) of each operand).  AK / BK: operand A (resp. B) is
+// contiguous along k; otherwise along m (resp. n), and its 8-element chunks
+// are transposed into LDS.  The k-tile is KK wide; the next one is fetched
+// into registers while the current one is multiplied.  The order of the
+// products over k does not depend on TN or KK.
 typedef short bf16x8 __attribute__((ext_vector_type(8)));
-constexpr int kT2 = 128, kK2 = 32, kLds2 = kK2 + 8;
+#ifndef EMT6RO_GEMM_KK
+#define EMT6RO_GEMM_KK 128
+#endif
+constexpr int kT2 = 128, kK2 = EMT6RO_GEMM_KK;
 
-template <bool AK>
+template <bool AK, int ROWS, int KK>
 __device__ __forceinline__ void load_chunks(const uint16_t *P, int64_t s_mn, int64_t s_k, int mn0,
-                                            int k0, int MN, int ke, uint4 v[2]) {
+                                            int k0, int MN, int ke, uint4 *v) {
+  constexpr int NC = ROWS * KK / 8 / 256;   // 8-element chunks per thread
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    const int c = (int)threadIdx.x + r * 256;  // 512 chunks of 8 elements
+  for (int r = 0; r < NC; ++r) {
+    const int c = (int)threadIdx.x + r * 256;
     int mn, k;
-    if (AK) { mn = mn0 + c / 4; k = k0 + (c % 4) * 8; }
-    else { k = k0 + c / 16; mn = mn0 + (c % 16) * 8; }
-    const bool ok = AK ? (mn < MN && k < ke) : (mn < MN && k < ke);
+    if (AK) { mn = mn0 + c / (KK / 8); k = k0 + (c % (KK / 8)) * 8; }
+    else { k = k0 + c / (ROWS / 8); mn = mn0 + (c % (ROWS / 8)) * 8; }
+    const bool ok = mn < MN && k < ke;
     v[r] = ok ? *reinterpret_cast<const uint4 *>(P + mn * s_mn + k * s_k) : make_uint4(0, 0, 0, 0);
   }
 }
 
-template <bool AK>
-__device__ __forceinline__ void store_chunks(uint16_t (*T)[kLds2], const uint4 v[2]) {
+template <bool AK, int ROWS, int KK>
+__device__ __forceinline__ void store_chunks(uint16_t (*T)[KK + 8], const uint4 *v) {
+  constexpr int NC = ROWS * KK / 8 / 256;
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
+  for (int r = 0; r < NC; ++r) {
     const int c = (int)threadIdx.x + r * 256;
     if (AK) {
-      *reinterpret_cast<uint4 *>(&T[c / 4][(c % 4) * 8]) = v[r];
+      *reinterpret_cast<uint4 *>(&T[c / (KK / 8)][(c % (KK / 8)) * 8]) = v[r];
     } else {
-      const int k = c / 16, mn = (c % 16) * 8;
+      const int k = c / (ROWS / 8), mn = (c % (ROWS / 8)) * 8;
       const uint16_t *e = reinterpret_cast<const uint16_t *>(&v[r]);
 #pragma unroll
       for (int i = 0; i < 8; ++i) T[mn + i][k] = e[i];
@@ -182,60 +212,74 @@ __device__ __forceinline__ void store_chunks(uint16_t (*T)[kLds2], const uint4 v
   }
 }
 
-template <bool AK, bool BK>
+template <bool AK, bool BK, int TM, int TN>
 __global__ __launch_bounds__(256) void gemm128_kernel(GemmArgs g) {
-  __shared__ __align__(16) uint16_t As[kT2][kLds2];
-  __shared__ __align__(16) uint16_t Bs[kT2][kLds2];
+  constexpr int KK = kK2, XB = TM / 64, YB = TN / 64;
+  constexpr int NA = TM * KK / 8 / 256, NB = TN * KK / 8 / 256;
+  __shared__ __align__(16) uint16_t As[TM][KK + 8];
+  __shared__ __align__(16) uint16_t Bs[TN][KK + 8];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m0 = blockIdx.y * kT2, n0 = blockIdx.x * kT2;
-  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int wm = (wave >> 1) * (TM / 2), wn = (wave & 1) * (TN / 2);
   const int kb = blockIdx.z * g.k_chunk;
   const int ke = min(g.K, kb + g.k_chunk);
-  f32x16 acc[2][2];
+  f32x16 acc[XB][YB];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < XB; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < YB; ++b)
 #pragma unroll
       for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.f;
   // A(m, k) = A[m sam + k sak]; B(k, n) = B[k sbk + n sbn].
-  uint4 ra[2], rb[2];
-  load_chunks<AK>(g.A, g.sam, g.sak, m0, kb, g.M, ke, ra);
-  load_chunks<BK>(g.B, g.sbn, g.sbk, n0, kb, g.N, ke, rb);
-  for (int k0 = kb; k0 < ke; k0 += kK2) {
+  uint4 ra[NA], rb[NB];
+  load_chunks<AK, TM, KK>(g.A, g.sam, g.sak, m0, kb, g.M, ke, ra);
+  load_chunks<BK, TN, KK>(g.B, g.sbn, g.sbk, n0, kb, g.N, ke, rb);
+  for (int k0 = kb; k0 < ke; k0 += KK) {
     __syncthreads();
-    store_chunks<AK>(As, ra);
-    store_chunks<BK>(Bs, rb);
+    store_chunks<AK, TM, KK>(As, ra);
+    store_chunks<BK, TN, KK>(Bs, rb);
     __syncthreads();
-    if (k0 + kK2 < ke) {
-      load_chunks<AK>(g.A, g.sam, g.sak, m0, k0 + kK2, g.M, ke, ra);
-      load_chunks<BK>(g.B, g.sbn, g.sbk, n0, k0 + kK2, g.N, ke, rb);
+    if (k0 + KK < ke) {
+      load_chunks<AK, TM, KK>(g.A, g.sam, g.sak, m0, k0 + KK, g.M, ke, ra);
+      load_chunks<BK, TN, KK>(g.B, g.sbn, g.sbk, n0, k0 + KK, g.N, ke, rb);
     }
 #pragma unroll
-    for (int kk = 0; kk < kK2; kk += 16) {
+    for (int kk = 0; kk < KK; kk += 16) {
       const int kq = kk + (lane >> 5) * 8;
-      bf16x8 a[2], b[2];
+      bf16x8 a[XB], b[YB];
 #pragma unroll
-      for (int x = 0; x < 2; ++x) {
+      for (int x = 0; x < XB; ++x)
         a[x] = *reinterpret_cast<const bf16x8 *>(&As[wm + x * 32 + (lane & 31)][kq]);
-        b[x] = *reinterpret_cast<const bf16x8 *>(&Bs[wn + x * 32 + (lane & 31)][kq]);
-      }
 #pragma unroll
-      for (int x = 0; x < 2; ++x)
+      for (int y = 0; y < YB; ++y)
+        b[y] = *reinterpret_cast<const bf16x8 *>(&Bs[wn + y * 32 + (lane & 31)][kq]);
 #pragma unroll
-        for (int y = 0; y < 2; ++y)
+      for (int x = 0; x < XB; ++x)
+#pragma unroll
+        for (int y = 0; y < YB; ++y)
           acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[x], b[y], acc[x][y], 0, 0, 0);
     }
   }
-  // acc[x][y][j]: row 8 (j / 4) + 4 (lane / 32) + j % 4 of block x, column lane % 32 of block y.
+  // acc[x][y][j]: row 8 (j / 4) + 4 (lane / 32) + j % 4 of block x, column
+  // lane % 32 of block y; the 4 rows of each j / 4 go to the transposed copy
+  // as one 8-byte store.
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int x = 0; x < XB; ++x)
 #pragma unroll
-    for (int y = 0; y < 2; ++y)
+    for (int y = 0; y < YB; ++y)
 #pragma unroll
-      for (int j = 0; j < 16; ++j)
-        gemm_store(g, m0 + wm + x * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3),
-                   n0 + wn + y * 32 + (lane & 31), acc[x][y][j]);
+      for (int q = 0; q < 4; ++q) {
+        const int mb = m0 + wm + x * 32 + 8 * q + 4 * (lane >> 5);
+        const int n = n0 + wn + y * 32 + (lane & 31);
+        uint16_t hv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) hv[i] = gemm_store(g, mb + i, n, acc[x][y][4 * q + i]);
+        if (g.ht_out && mb < g.M && n < g.N) {
+          const uint2 w2 = make_uint2((uint32_t)hv[0] | ((uint32_t)hv[1] << 16),
+                                      (uint32_t)hv[2] | ((uint32_t)hv[3] << 16));
+          *reinterpret_cast<uint2 *>(g.ht_out + (int64_t)n * g.ldt + mb) = w2;
+        }
+      }
 }
 
 // y[b] = sum_k h[b, k] w[k] + bias   (one wave per row)
@@ -259,12 +303,47 @@ __global__ void loss_kernel(const float *y, const float *t, float *dy, float *sq
 }
 
 // dz[b, k] = dy[b] w[k] gelu'(z[b, k])  (bf16)
-__global__ void out_back_kernel(const float *dy, const float *w, const uint16_t *z, uint16_t *dz,
-                                int B, int K) {
+// A 64 x 64 tile of a row-major bf16 result, written to out (row-major,
+// ld cols) and through LDS to out_t (its transpose, ld rows), both with
+// coalesced stores.  value(r, c) gives the element; 256 threads.
+template <typename F>
+__device__ __forceinline__ void tile_store_both(int r0, int c0, int rows, int cols,
+                                                uint16_t *out, uint16_t *out_t, const F &value) {
+  __shared__ uint16_t tile[64][66];
+  const int tid = threadIdx.x;
+#pragma unroll 4
+  for (int q = tid; q < 64 * 64; q += 256) {
+    const int r = q >> 6, c = q & 63;
+    if (r0 + r < rows && c0 + c < cols) {
+      const uint16_t v = value(r0 + r, c0 + c);
+      out[(int64_t)(r0 + r) * cols + c0 + c] = v;
+      tile[r][c] = v;
+    }
+  }
+  if (!out_t) return;
+  __syncthreads();
+#pragma unroll 4
+  for (int q = tid; q < 64 * 64; q += 256) {
+    const int c = q >> 6, r = q & 63;
+    if (r0 + r < rows && c0 + c < cols) out_t[(int64_t)(c0 + c) * rows + r0 + r] = tile[r][c];
+  }
+}
+
+// dz = dy w4 gelu'(z3) (and its transpose), one 64 x 64 tile per block.
+__global__ __launch_bounds__(256) void out_back_kernel(const float *dy, const float *w,
+                                                       const uint16_t *z, uint16_t *dz,
+                                                       uint16_t *dzt, int B, int K) {
+  tile_store_both(blockIdx.y * 64, blockIdx.x * 64, B, K, dz, dzt, [&](int b, int k) {
+    return to_bf16(dy[b] * w[k] * gelu_grad(from_bf16(z[(int64_t)b * K + k])));
+  });
+}
+
+// yt[c * rows + r] = x[r * cols + c] (bf16).
+__global__ void transpose_bf16_kernel(const uint16_t *x, int rows, int cols, uint16_t *yt) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)B * K) return;
-  const int b = (int)(i / K), k = (int)(i % K);
-  dz[i] = to_bf16(dy[b] * w[k] * gelu_grad(from_bf16(z[i])));
+  if (i >= (int64_t)rows * cols) return;
+  const int r = (int)(i / cols), c = (int)(i % cols);
+  yt[(int64_t)c * rows + r] = x[i];
 }
 
 // out[i] = sum over s < nsplit of part[s * stride + i] (fixed order).
@@ -298,6 +377,38 @@ __global__ __launch_bounds__(256) void colsum_part_kernel(const uint16_t *x, con
         ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
 }
 
+// Batched forms of the two kernels above for the training step: blockIdx.z
+// (column sums) or blockIdx.y (slice sums) picks one of up to kMaxJobs jobs,
+// each summed exactly as the single-job kernels do.
+__global__ __launch_bounds__(256) void colsum_part_multi_kernel(ColsumJobs j, int B) {
+  const ColsumJob &c = j.job[blockIdx.z];
+  if ((int)blockIdx.x * 64 >= c.K) return;
+  __shared__ float red[4][64];
+  const int K = c.K;
+  const int k = blockIdx.x * 64 + (threadIdx.x & 63), r0 = threadIdx.x >> 6;
+  const int b0 = blockIdx.y * kColRows, b1 = min(B, b0 + kColRows);
+  float sum = 0.f;
+  if (k < K)
+    for (int b = b0 + r0; b < b1; b += 4) {
+      const float v = from_bf16(c.x[(int64_t)b * K + k]);
+      sum += c.rowscale ? c.rowscale[b] * v : v;
+    }
+  red[r0][threadIdx.x & 63] = sum;
+  __syncthreads();
+  if (r0 == 0 && k < K)
+    c.part[(int64_t)blockIdx.y * K + k] =
+        ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+}
+
+__global__ void sum_slices_multi_kernel(SliceJobs j) {
+  const SliceJob &c = j.job[blockIdx.y];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.n) return;
+  float sum = c.part[i];
+  for (int z = 1; z < c.nsplit; ++z) sum += c.part[(int64_t)z * c.stride + i];
+  c.out[i] = sum;
+}
+
 // Adam (Kingma & Ba, 2015) with bias correction; refreshes the bf16 copy.
 __global__ void adam_kernel(float *w, float *m, float *v, const float *g, uint16_t *wb, int n,
                             float lr, float b1, float b2, float eps, float c1, float c2) {
@@ -311,6 +422,33 @@ __global__ void adam_kernel(float *w, float *m, float *v, const float *g, uint16
   const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
   w[i] = wi;
   if (wb) wb[i] = to_bf16(wi);
+}
+
+// Adam over the whole flat parameter vector in one launch; elements of the
+// kAdamBf16 segments [off, off + n) also refresh their bf16 copies.
+__global__ void adam_all_kernel(float *w, float *m, float *v, const float *g, AdamBf16 seg,
+                                int n, float lr, float b1, float b2, float eps, float c1,
+                                float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+#pragma unroll
+  for (int k = 0; k < kAdamBf16; ++k)
+    if (i >= seg.off[k] && i < seg.off[k] + seg.n[k]) {
+      const int e = i - seg.off[k];
+      const uint16_t b = to_bf16(wi);
+      seg.wb[k][e] = b;
+      if (seg.wbt[k]) {
+        const int r = e / seg.cols[k], c = e - r * seg.cols[k];
+        seg.wbt[k][(int64_t)c * (seg.n[k] / seg.cols[k]) + r] = b;
+      }
+    }
 }
 
 __global__ void f32_to_bf16_kernel(const float *x, uint16_t *y, int64_t n) {
@@ -329,14 +467,22 @@ __device__ __forceinline__ int32_t pick_row(uint64_t seed, uint32_t step, uint32
 
 // Rows of a dataset (bf16, ld columns) gathered into a minibatch; the rows
 // are drawn on the device, so a training step needs no host round trip.
-__global__ void gather_kernel(const uint16_t *X, const float *t, uint64_t seed, uint32_t step,
-                              int n, uint16_t *Xb, float *tb, int B, int ld) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)B * ld) return;
-  const int b = (int)(i / ld), c = (int)(i % ld);
-  const int r = pick_row(seed, step, (uint32_t)b, n);
-  Xb[i] = X[(int64_t)r * ld + c];
-  if (c == 0) tb[b] = t[r];
+// One 64-row x 64-column tile of the minibatch (and its transpose).
+__global__ __launch_bounds__(256) void gather_kernel(const uint16_t *X, const float *t,
+                                                     uint64_t seed, uint32_t step, int n,
+                                                     uint16_t *Xb, uint16_t *Xbt, float *tb,
+                                                     int B, int ld) {
+  const int b0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  __shared__ int32_t src[64];
+  if (threadIdx.x < 64 && b0 + (int)threadIdx.x < B) {
+    const int b = b0 + threadIdx.x;
+    src[threadIdx.x] = pick_row(seed, step, (uint32_t)b, n);
+    if (blockIdx.x == 0) tb[b] = t[src[threadIdx.x]];
+  }
+  __syncthreads();
+  tile_store_both(b0, c0, B, ld, Xb, Xbt, [&](int b, int c) {
+    return X[(int64_t)src[b - b0] * ld + c];
+  });
 }
 
 // out[0] = mean of sq[0..B) (one workgroup).
@@ -362,6 +508,19 @@ namespace {
 inline int blocks(int64_t n, int t) { return (int)((n + t - 1) / t); }
 }  // namespace
 
+#ifndef EMT6RO_GEMM_MIN_GROUPS
+#define EMT6RO_GEMM_MIN_GROUPS 512
+#endif
+constexpr int kMinGroups = EMT6RO_GEMM_MIN_GROUPS;
+
+template <int TM, int TN>
+void launch_wide(bool a_k, bool b_k, dim3 grid, const mlp::GemmArgs &g, hipStream_t s) {
+  if (a_k && b_k) hipLaunchKernelGGL((mlp::gemm128_kernel<true, true, TM, TN>), grid, dim3(256), 0, s, g);
+  else if (a_k) hipLaunchKernelGGL((mlp::gemm128_kernel<true, false, TM, TN>), grid, dim3(256), 0, s, g);
+  else if (b_k) hipLaunchKernelGGL((mlp::gemm128_kernel<false, true, TM, TN>), grid, dim3(256), 0, s, g);
+  else hipLaunchKernelGGL((mlp::gemm128_kernel<false, false, TM, TN>), grid, dim3(256), 0, s, g);
+}
+
 hipError_t mlp_gemm(const MlpGemm &q, hipStream_t s) {
   mlp::GemmArgs g{};
   g.A = q.A; g.B = q.B;
@@ -369,38 +528,59 @@ hipError_t mlp_gemm(const MlpGemm &q, hipStream_t s) {
   g.M = q.M; g.N = q.N; g.K = q.K;
   g.epi = q.epi; g.bias = q.bias; g.act = q.act;
   g.z_out = q.z_out; g.h_out = q.h_out; g.z_in = q.z_in; g.f_out = q.f_out; g.ldc = q.ldc;
-  const int tiles = ((q.N + mlp::kTile - 1) / mlp::kTile) * ((q.M + mlp::kTile - 1) / mlp::kTile);
+  g.ht_out = q.ht_out; g.ldt = q.ldt;
+  const bool a_k = q.sak == 1, b_k = q.sbk == 1;
+  const int64_t sa = a_k ? q.sam : q.sak, sb = b_k ? q.sbn : q.sbk;
+  const bool wide = q.M % 8 == 0 && q.N % 8 == 0 && q.K % 8 == 0 && sa % 8 == 0 &&
+                    sb % 8 == 0 && (a_k || q.sam == 1) && (b_k || q.sbn == 1) &&
+                    ((uintptr_t)q.A & 15) == 0 && ((uintptr_t)q.B & 15) == 0;
+  // Tiles: on the wide path the largest of 128 x 128, 128 x 64 and 64 x 64
+  // that still gives kMinGroups workgroups (two per CU), else 64 x 64; the
+  // 64 x 64 kernel of the narrow path otherwise.
+  int tm = wide ? mlp::kT2 : mlp::kTile;
+  int tn = wide ? mlp::kT2 : mlp::kTile;
+  auto ntiles = [&](int tn_) { return ((q.N + tn_ - 1) / tn_) * ((q.M + tm - 1) / tm); };
   // Split a long reduction over blockIdx.z when the output has few tiles
   // (the weight gradients: 512 x 512 outputs, 4096-row batches); the
   // slices land in `work` and are summed in a fixed order.
-  int split = 1;
-  if (q.epi == mlp::kEpiStoreF32 && q.work && q.work_floats > 0)
-    while (split < 16 && tiles * split * 2 <= 1024 && q.K / (split * 2) >= 4 * mlp::kTK &&
-           (int64_t)(split * 2) * q.M * q.ldc <= q.work_floats)
-      split *= 2;
-  g.k_chunk = ((q.K + split - 1) / split + mlp::kTK - 1) / mlp::kTK * mlp::kTK;
+  const int kq = wide ? mlp::kK2 : mlp::kTK;
+  auto nsplit = [&](int tiles) {
+    int sp = 1;
+    if (q.epi == mlp::kEpiStoreF32 && q.work && q.work_floats > 0)
+      while (sp < 16 && tiles * sp * 2 <= 1024 && q.K / (sp * 2) >= 4 * kq &&
+             (int64_t)(sp * 2) * q.M * q.ldc <= q.work_floats)
+        sp *= 2;
+    return sp;
+  };
+  int split = nsplit(ntiles(tn));
+  if (wide && ntiles(tn) * split < kMinGroups) {
+    tn = 64;
+    split = nsplit(ntiles(tn));
+  }
+  if (wide && ntiles(tn) * split < kMinGroups) {
+    tm = 64;
+    split = nsplit(ntiles(tn));
+  }
+  g.k_chunk = ((q.K + split - 1) / split + kq - 1) / kq * kq;
   split = (q.K + g.k_chunk - 1) / g.k_chunk;
   if (split > 1) {
     g.f_out = q.work;
     g.split_stride = (int64_t)q.M * q.ldc;
   }
-  const bool a_k = q.sak == 1, b_k = q.sbk == 1;
-  const int64_t sa = a_k ? q.sam : q.sak, sb = b_k ? q.sbn : q.sbk;
-  const bool wide = q.M % 8 == 0 && q.N % 8 == 0 && q.K % 8 == 0 && sa % 8 == 0 &&
-                    sb % 8 == 0 && (a_k || q.sam == 1) && (b_k || q.sbn == 1) &&
-                    ((uintptr_t)q.A & 15) == 0 && ((uintptr_t)q.B & 15) == 0 &&
-                    g.k_chunk % mlp::kK2 == 0;
   if (wide) {
-    dim3 grid2((q.N + mlp::kT2 - 1) / mlp::kT2, (q.M + mlp::kT2 - 1) / mlp::kT2, split);
-    if (a_k && b_k) hipLaunchKernelGGL((mlp::gemm128_kernel<true, true>), grid2, dim3(256), 0, s, g);
-    else if (a_k) hipLaunchKernelGGL((mlp::gemm128_kernel<true, false>), grid2, dim3(256), 0, s, g);
-    else if (b_k) hipLaunchKernelGGL((mlp::gemm128_kernel<false, true>), grid2, dim3(256), 0, s, g);
-    else hipLaunchKernelGGL((mlp::gemm128_kernel<false, false>), grid2, dim3(256), 0, s, g);
+    dim3 grid2((q.N + tn - 1) / tn, (q.M + tm - 1) / tm, split);
+    if (tm == 64)
+      launch_wide<64, 64>(a_k, b_k, grid2, g, s);
+    else if (tn == 64)
+      launch_wide<128, 64>(a_k, b_k, grid2, g, s);
+    else
+      launch_wide<128, 128>(a_k, b_k, grid2, g, s);
   } else {
     dim3 grid((q.N + mlp::kTile - 1) / mlp::kTile, (q.M + mlp::kTile - 1) / mlp::kTile, split);
     hipLaunchKernelGGL(mlp::gemm_bf16_kernel, grid, dim3(256), 0, s, g);
   }
-  if (split > 1) {
+  if (q.split_out) *q.split_out = split;
+  if (split > 1 && !q.defer_sum) {
     const int64_t n = (int64_t)q.M * q.ldc;
     hipLaunchKernelGGL(mlp::sum_slices_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, q.work,
                        n, split, q.f_out, n);
@@ -419,10 +599,16 @@ hipError_t mlp_loss(const float *y, const float *t, float *dy, float *sq, int B,
   return hipGetLastError();
 }
 
-hipError_t mlp_out_back(const float *dy, const float *w, const uint16_t *z, uint16_t *dz, int B,
-                        int K, hipStream_t s) {
-  hipLaunchKernelGGL(mlp::out_back_kernel, dim3(blocks((int64_t)B * K, 256)), dim3(256), 0, s, dy, w,
-                     z, dz, B, K);
+hipError_t mlp_out_back(const float *dy, const float *w, const uint16_t *z, uint16_t *dz,
+                        uint16_t *dzt, int B, int K, hipStream_t s) {
+  hipLaunchKernelGGL(mlp::out_back_kernel, dim3(blocks(K, 64), blocks(B, 64)), dim3(256), 0, s, dy,
+                     w, z, dz, dzt, B, K);
+  return hipGetLastError();
+}
+
+hipError_t mlp_transpose_bf16(const uint16_t *x, int rows, int cols, uint16_t *yt, hipStream_t s) {
+  hipLaunchKernelGGL(mlp::transpose_bf16_kernel, dim3(blocks((int64_t)rows * cols, 256)), dim3(256),
+                     0, s, x, rows, cols, yt);
   return hipGetLastError();
 }
 
@@ -433,6 +619,41 @@ hipError_t mlp_colsum(const uint16_t *x, const float *rowscale, float *out, int 
                      rowscale, work, B, K);
   hipLaunchKernelGGL(mlp::sum_slices_kernel, dim3(blocks(K, 256)), dim3(256), 0, s, work,
                      (int64_t)K, nb, out, (int64_t)K);
+  return hipGetLastError();
+}
+
+hipError_t mlp_colsum_multi(const ColsumJobs &jobs, int njobs, int B, hipStream_t s) {
+  if (njobs <= 0 || njobs > kMaxJobs) return hipErrorInvalidValue;
+  int kmax = 0;
+  SliceJobs sj{};
+  const int nb = (B + mlp::kColRows - 1) / mlp::kColRows;
+  for (int i = 0; i < njobs; ++i) {
+    kmax = std::max(kmax, jobs.job[i].K);
+    sj.job[i] = SliceJob{jobs.job[i].part, (int64_t)jobs.job[i].K, nb, jobs.job[i].out,
+                         (int64_t)jobs.job[i].K};
+  }
+  hipLaunchKernelGGL(mlp::colsum_part_multi_kernel, dim3(blocks(kmax, 64), nb, njobs), dim3(256), 0,
+                     s, jobs, B);
+  return mlp_sum_slices_multi(sj, njobs, s);
+}
+
+hipError_t mlp_sum_slices_multi(const SliceJobs &jobs, int njobs, hipStream_t s) {
+  if (njobs <= 0 || njobs > kMaxJobs) return hipErrorInvalidValue;
+  int64_t nmax = 0;
+  for (int i = 0; i < njobs; ++i) nmax = std::max(nmax, jobs.job[i].n);
+  hipLaunchKernelGGL(mlp::sum_slices_multi_kernel, dim3(blocks(nmax, 256), njobs), dim3(256), 0, s,
+                     jobs);
+  return hipGetLastError();
+}
+
+hipError_t mlp_adam_all(float *w, float *m, float *v, const float *g, const AdamBf16 &seg, int n,
+                        float lr, float b1, float b2, float eps, int t, hipStream_t s) {
+  float c1 = 1.f, c2 = 1.f;
+  for (int i = 0; i < t; ++i) { c1 *= b1; c2 *= b2; }
+  c1 = 1.f - c1;
+  c2 = 1.f - c2;
+  hipLaunchKernelGGL(mlp::adam_all_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, w, m, v, g, seg, n,
+                     lr, b1, b2, eps, c1, c2);
   return hipGetLastError();
 }
 
@@ -453,9 +674,9 @@ hipError_t mlp_to_bf16(const float *x, uint16_t *y, int64_t n, hipStream_t s) {
 }
 
 hipError_t mlp_gather(const uint16_t *X, const float *t, uint64_t seed, uint32_t step, int n,
-                      uint16_t *Xb, float *tb, int B, int ld, hipStream_t s) {
-  hipLaunchKernelGGL(mlp::gather_kernel, dim3(blocks((int64_t)B * ld, 256)), dim3(256), 0, s, X, t,
-                     seed, step, n, Xb, tb, B, ld);
+                      uint16_t *Xb, uint16_t *Xbt, float *tb, int B, int ld, hipStream_t s) {
+  hipLaunchKernelGGL(mlp::gather_kernel, dim3(blocks(ld, 64), blocks(B, 64)), dim3(256), 0, s, X, t,
+                     seed, step, n, Xb, Xbt, tb, B, ld);
   return hipGetLastError();
 }
 

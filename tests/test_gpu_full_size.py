@@ -132,3 +132,70 @@ def test_uneven_workload_first_hour():
         got = arrays_of(b.state(i))
         for name, a in states_p[i].items():
             assert np.array_equal(got[name].view(np.uint8), a.view(np.uint8)), (i, name)
+
+
+def _to_horizon(wl, idx, seed, label):
+    """Simulations `idx` of a workload (their own streams, tumours,
+    protocols and horizons) run to their horizons on the persistent path,
+    each checked bit for bit against the oracle over the whole run (the
+    oracle runs the simulations of each lattice size in parallel)."""
+    import emt6ro.simulation as sim
+    from emt6ro.batch import Batch
+    params = sim.load_parameters(golden("default-parameters.json"))
+    v = params.values()
+    idx = np.asarray(idx)
+    b = Batch(params, wl["tumors"], np.asarray(wl["tumor_index"])[idx],
+              np.asarray(wl["prot_index"])[idx], np.asarray(wl["horizon"])[idx], wl["doses"],
+              300, wl["prot_length"], sim_id=idx, seed=seed)
+    hmax = int(np.asarray(wl["horizon"])[idx].max())
+    log(f"[{label}] persistent path to the horizon ({hmax} steps) for {len(idx)} simulations")
+    b.run(hmax, path="persistent")
+    res = b.results().astype(np.int64)
+    cnt = b.counters()
+    assert np.array_equal(cnt["step"], np.asarray(wl["horizon"])[idx]) and (cnt["status"] == 0).all()
+    dims = [wl["tumors"][int(wl["tumor_index"][i])].shape for i in idx]
+    for shape in sorted(set(dims)):
+        # One oracle batch per lattice size, run in stages to each horizon;
+        # a simulation is checked when the batch reaches its horizon.
+        grp = [k for k in range(len(idx)) if dims[k] == shape]
+        states = [oracle_state_from_arrays(
+            arrays_of(wl["tumors"][int(wl["tumor_index"][idx[k]])]), v) for k in grp]
+        orc = O.OracleBatch(v, states, wl["doses"], [int(wl["prot_index"][idx[k]]) for k in grp],
+                            [int(idx[k]) for k in grp], 300, wl["prot_length"])
+        done = 0
+        for h in sorted(set(int(wl["horizon"][idx[k]]) for k in grp)):
+            log(f"[{label}] oracle: {len(grp)} simulations of {shape[0]}x{shape[1]} to step {h}")
+            orc.run(h - done, seed=seed, nthreads=16)
+            done = h
+            for j, k in enumerate(grp):
+                if int(wl["horizon"][idx[k]]) != h:
+                    continue
+                assert res[k] == orc.counts()[j], (int(idx[k]), res[k], orc.counts()[j])
+                compare_states(b.state(k), orc, j, f"{label} sim {int(idx[k])} at step {h}")
+    log(f"[{label}] living at the horizons: {res.tolist()}")
+
+
+def test_uniform_workload_to_144h_matches_oracle():
+    """Simulations spread over the 20,000 of the uniform workload, run to
+    their 144 h horizon; the batch keeps the window pass, whose ROIs grow out
+    of the window during the run."""
+    import emt6ro.simulation as sim
+    from emt6ro.workloads import batched_uniform
+    params = sim.load_parameters(golden("default-parameters.json"))
+    wl = batched_uniform(params, seed=0)
+    _to_horizon(wl, [0, 19, 4321, 9999, 15000, 19999], 0, "uniform 144 h")
+
+
+def test_uneven_workload_to_horizons_matches_oracle():
+    """Two simulations of every (lattice size, horizon) pair of the 50,000
+    mixed workload (33/53/81, 48/144/240 h) run to their horizons."""
+    import emt6ro.simulation as sim
+    from emt6ro.workloads import batched_uneven
+    params = sim.load_parameters(golden("default-parameters.json"))
+    wl = batched_uneven(params, seed=0)
+    idx = []
+    for size in (33, 53, 81):
+        for h in (48, 144, 240):
+            cand = np.flatnonzero((wl["sizes"] == size) & (wl["hours"] == h))
+            idx += [int(cand[0]), int(cand[len(cand) // 2])]
+    _to_horizon(wl, idx, 3, "uneven to horizons")
