@@ -572,3 +572,25 @@ def test_division_into_a_site_freed_in_the_same_step(rules, target_later):
     assert divided == (rules == 0 or not target_later)
     assert b.st[0][target] == (O.OCCUPIED if divided else O.VACANT)
     assert cell_at(b, 10, 10)["phase"] == (G1 if divided else D)
+
+
+def test_literal_q3_q4_rules_ab_on_config1():
+    """tests/golden/rules_ab.json: the config-1 workload (144 h) under the
+    deterministic rules (0) and the reference's literal Q3/Q4 behaviour (3),
+    1,000 paired replicates each (same streams).  The living-count
+    distributions are not distinguishable: KS p > 0.05, paired mean
+    difference within 3 standard errors and under 1% of the mean."""
+    import json
+    from scipy import stats
+    d = json.load(open(golden("rules_ab.json")))
+    a = np.asarray(d["rules_0"]["counts"], np.float64)
+    b = np.asarray(d["rules_3"]["counts"], np.float64)
+    assert len(a) == len(b) == 1000
+    assert d["workload"]["steps"] == 86400 and d["workload"]["size"] == 53
+    assert abs(a.mean() - d["rules_0"]["mean"]) < 1e-9
+    diff = b - a
+    se = diff.std(ddof=1) / np.sqrt(len(diff))
+    assert stats.ks_2samp(a, b).pvalue > 0.05
+    assert abs(diff.mean()) < 3 * se and abs(diff.mean()) < 0.01 * a.mean()
+    # Paired: the same streams give strongly related outcomes.
+    assert np.corrcoef(a, b)[0, 1] > 0.5
