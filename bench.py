@@ -74,6 +74,9 @@ def parse():
                     help="skip the surrogate training / screening sample")
     ap.add_argument("--surrogate-only", action="store_true",
                     help="only run the surrogate sample (profiling runs)")
+    ap.add_argument("--surrogate-graph", type=int, default=None,
+                    help="training steps per replayed HIP graph (0: direct launches; "
+                         "default: the library's)")
     ap.add_argument("--json-out", default=None)
     return ap.parse_args()
 
@@ -146,7 +149,8 @@ def cpu_baseline(params, wl, hours):
             "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "cpus_available": avail}
 
 
-def surrogate_sample(train_steps=200, batch=4096, n_data=200000, n_screen=100000):
+def surrogate_sample(train_steps=200, batch=4096, n_data=200000, n_screen=100000,
+                     graph_steps=None):
     """The fitness surrogate's network (N_FEATURES-512-512-512-1, BF16 on the
     matrix cores): Adam steps on random data and one screening pass, timed
     on the host."""
@@ -157,11 +161,23 @@ def surrogate_sample(train_steps=200, batch=4096, n_data=200000, n_screen=100000
     X = rng.standard_normal((n_data, N_FEATURES)).astype(np.float32)
     y = rng.standard_normal(n_data).astype(np.float32)
     net = MLP(N_FEATURES, hidden, seed=0)
+    if graph_steps is not None:
+        net.set_train_graph(graph_steps)
     net.set_data(X, y)
     net.train(10, batch=batch, lr=1e-4, seed=1)            # warm-up
     t0 = time.perf_counter()
     net.train(train_steps, batch=batch, lr=1e-4, seed=2)
     train_s = time.perf_counter() - t0
+    # The GA's surrogate is an ensemble of 5 such networks, trained together
+    # (train_many: their steps interleaved on their own streams).
+    from emt6ro.surrogate import train_many
+    ens = [net] + [MLP(N_FEATURES, hidden, seed=1 + m) for m in range(4)]
+    for m in ens[1:]:
+        m.set_data(X, y)
+    train_many(ens, 10, batch=batch, lr=1e-4, seeds=range(5))      # warm-up
+    t0 = time.perf_counter()
+    train_many(ens, train_steps, batch=batch, lr=1e-4, seeds=range(10, 15))
+    ens_s = time.perf_counter() - t0
     Xs = rng.standard_normal((n_screen, N_FEATURES)).astype(np.float32)
     net.predict(Xs[:4096])
     t0 = time.perf_counter()
@@ -174,6 +190,8 @@ def surrogate_sample(train_steps=200, batch=4096, n_data=200000, n_screen=100000
     step_flop = 2 * fwd + 2 * (2 * hidden * hidden) * batch
     return {"train_steps_per_s": train_steps / train_s,
             "train_tflops_mfma_gemm": step_flop * train_steps / train_s / 1e12,
+            "ensemble5_member_steps_per_s": 5 * train_steps / ens_s,
+            "ensemble5_tflops_mfma_gemm": step_flop * 5 * train_steps / ens_s / 1e12,
             "batch": batch, "screen_rows_per_s": n_screen / screen_s,
             "note": "host-timed; includes the host<->device copies of predict"}
 
@@ -216,7 +234,8 @@ def main():
         copy_calibration()
         return
     if args.surrogate_only:
-        print(json.dumps({"surrogate": surrogate_sample()}), flush=True)
+        print(json.dumps({"surrogate": surrogate_sample(graph_steps=args.surrogate_graph)}),
+              flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the simulations are "
@@ -346,7 +365,7 @@ def main():
         log(f"[bench] per-step path sample: {p_ms / len(sampled):.0f} ms/h vs "
             f"{q_ms / len(sampled):.0f} ms/h persistent")
     if not args.no_surrogate:
-        sur = surrogate_sample()
+        sur = surrogate_sample(graph_steps=args.surrogate_graph)
         out["surrogate"] = sur
         # One GA fitness evaluation = 3 tumours x 20 replicates x 144 h.
         plain_evals = value / (3 * 20 * HOURS)

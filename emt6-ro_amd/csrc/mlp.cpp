@@ -52,6 +52,7 @@ class Mlp {
     // end of the backward pass).
     kwork_floats_ = 16 * std::max<int64_t>((int64_t)H_ * kin_, (int64_t)H_ * H_);
     kwork_ = dalloc<float>(kLayers * kwork_floats_);
+    clock_ = dalloc<TrainClock>(1);
     // PyTorch-style init: U(-1/sqrt(fan_in), 1/sqrt(fan_in)).
     std::mt19937_64 rng(seed);
     std::vector<float> h(tot, 0.f);
@@ -117,27 +118,104 @@ class Mlp {
 
   // Adam on minibatches drawn from the data set on the device; returns each
   // step's mean squared error.  The steps are enqueued back to back and the
-  // host waits once, at the end.
+  // host waits once, at the end.  With graph_steps_ > 0 the kernels of that
+  // many steps are captured once into a HIP graph, which is replayed; the
+  // remaining steps are enqueued directly.  Both orders run the same kernels
+  // on the same arguments: what changes from step to step (minibatch stream,
+  // loss slot, Adam bias corrections) is read from the device clock.
   std::vector<float> train(int steps, int batch, float lr, uint64_t seed) {
+    float *dl = train_begin(steps, batch);
+    if (!dl) return {};
+    int done = 0;
+    const int g = graph_steps_;
+    if (g > 0 && steps >= g) {
+      hipGraphExec_t exec = capture_steps(g, batch, lr, seed, dl);
+      hipError_t e = hipSuccess;
+      for (; e == hipSuccess && done + g <= steps; done += g) e = hipGraphLaunch(exec, s_);
+      if (e == hipSuccess) e = hipStreamSynchronize(s_);
+      (void)hipGraphExecDestroy(exec);
+      EMT_HIPM(e);
+    }
+    for (; done < steps; ++done) enqueue_step(batch, lr, seed, dl);
+    return train_end(dl, steps);
+  }
+
+  // The kernels of g training steps captured from this network's stream
+  // into an executable graph (the caller launches and destroys it).
+  hipGraphExec_t capture_steps(int g, int batch, float lr, uint64_t seed, float *dl) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    EMT_HIPM(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int k = 0; k < g; ++k) enqueue_step(batch, lr, seed, dl);
+    } catch (...) {
+      (void)hipStreamEndCapture(s_, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    EMT_HIPM(hipStreamEndCapture(s_, &graph));
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    EMT_HIPM(e);
+    return exec;
+  }
+
+  hipStream_t stream() const { return s_; }
+  int graph_steps() const { return graph_steps_; }
+
+  // train() in parts, so that several networks can interleave their steps
+  // on their own streams (emt6ro_mlp_train_many): train_begin() checks the
+  // arguments, sizes the batch buffers, allocates the loss slots and sets the
+  // device clock (null when there are no steps); enqueue_step() adds one
+  // step; train_end() waits for this network's stream and returns the losses.
+  float *train_begin(int steps, int batch) {
     EMT_HIPM(hipSetDevice(device_));
     EMT_ENFORCE(n_data_ > 0, -1, "set_data first");
     EMT_ENFORCE(steps >= 0 && batch > 0, -1, "steps >= 0 and batch > 0");
     ensure_batch(batch);
-    std::vector<float> losses(steps);
-    if (steps == 0) return losses;
+    if (steps == 0) return nullptr;
     float *dl = dalloc<float>(steps);
-    for (int it = 0; it < steps; ++it) {
-      EMT_HIPM(mlp_gather(data_x_, data_t_, seed, (uint32_t)it, n_data_, x_, xt_, t_, batch, kin_,
-                          s_));
-      forward(batch, true);
-      backward(batch);
-      adam(lr);
-      EMT_HIPM(mlp_mean(sq_, batch, dl + it, s_));
-    }
+    // b1^t and b2^t for the t Adam steps taken so far; each step advances
+    // them before its Adam update.
+    TrainClock c{0, 0, 1.f, 1.f};
+    for (int i = 0; i < step_; ++i) { c.p1 *= kB1; c.p2 *= kB2; }
+    EMT_HIPM(hipMemcpyAsync(clock_, &c, sizeof c, hipMemcpyHostToDevice, s_));
+    return dl;
+  }
+
+  std::vector<float> train_end(float *dl, int steps) {
+    std::vector<float> losses(steps);
+    step_ += steps;
     EMT_HIPM(hipMemcpyAsync(losses.data(), dl, (size_t)steps * 4, hipMemcpyDeviceToHost, s_));
     EMT_HIPM(hipStreamSynchronize(s_));
     release(dl);
     return losses;
+  }
+
+  int device() const { return device_; }
+
+  // One training step: minibatch, forward, backward (whose slice sums also
+  // store the loss and advance the device clock), Adam.
+  void enqueue_step(int batch, float lr, uint64_t seed, float *dl) {
+    EMT_HIPM(mlp_gather(data_x_, data_t_, seed, clock_, n_data_, x_, xt_, t_, batch, kin_, s_));
+    forward(batch, true);
+    backward(batch, dl);
+    AdamBf16 seg{};
+    for (int l = 0; l < kLayers; ++l) {
+      seg.wb[l] = wb_[l];
+      seg.wbt[l] = wbt_[l];
+      seg.off[l] = (int)offs_[2 * l];
+      seg.n[l] = (int)sizes_[2 * l];
+      seg.cols[l] = l == 0 ? kin_ : H_;
+    }
+    EMT_HIPM(mlp_adam_all(w_, m_, v_, g_, seg, (int)total_, lr, kB1, kB2, 1e-8f, clock_, s_));
+  }
+
+
+  // Steps per captured graph in train() (0: none).
+  void set_graph_steps(int g) {
+    EMT_ENFORCE(g >= 0 && g <= 1024, -1, "graph steps must be in [0, 1024]");
+    graph_steps_ = g;
   }
 
   void predict(const float *X, int n, float *y) {
@@ -234,7 +312,7 @@ class Mlp {
     EMT_HIPM(hipStreamSynchronize(s_));
     release(x_);
     for (int l = 0; l < kLayers; ++l) { release(z_[l]); release(h_[l]); release(dz_[l]); }
-    release(y_); release(t_); release(dy_); release(sq_); release(ones_); release(xs_);
+    release(y_); release(t_); release(dy_); release(sq_); release(xs_); release(ypart_);
     release(cwork_); release(xt_);
     for (int l = 0; l < kLayers; ++l) { release(ht_[l]); release(dzt_[l]); }
     bmax_ = B;
@@ -254,11 +332,10 @@ class Mlp {
       dz_[l] = dalloc<uint16_t>((size_t)B * H_);
     }
     y_ = dalloc<float>(B); t_ = dalloc<float>(B); dy_ = dalloc<float>(B); sq_ = dalloc<float>(B);
-    ones_ = dalloc<uint16_t>(B);
-    cwork_ = dalloc<float>((size_t)(2 + kLayers) * ((B + 127) / 128) * H_);
-    std::vector<uint16_t> one(B, 0x3f80);  // bf16 1.0
-    EMT_HIPM(hipMemcpyAsync(ones_, one.data(), B * 2, hipMemcpyHostToDevice, s_));
-    EMT_HIPM(hipStreamSynchronize(s_));
+    ypart_ = dalloc<float>((size_t)((H_ + 63) / 64) * B);
+    // Bias-gradient column sums per row block: 3 of ceil(B / 64) x H (output
+    // layer backward) and 2 of ceil(B / 32) x H (input-gradient epilogues).
+    cwork_ = dalloc<float>((size_t)5 * ((B + 31) / 32) * H_);
   }
 
   // Host rows (in_ columns) -> padded bf16 input x_, targets -> t_.
@@ -273,8 +350,11 @@ class Mlp {
     EMT_HIPM(hipStreamSynchronize(s_));
   }
 
-  // keep_t: also write the transposed layer outputs the backward pass needs.
+  // keep_t (training): also write the transposed layer outputs the backward
+  // pass needs, and leave the output layer to the backward pass when the
+  // last hidden layer's product can give its partial sums (ypart_ready_).
   void forward(int B, bool keep_t = false) {
+    ypart_ready_ = 0;
     const uint16_t *in = x_;
     int K = kin_;
     for (int l = 0; l < kLayers; ++l) {
@@ -285,18 +365,36 @@ class Mlp {
       q.epi = 0; q.bias = w_ + offs_[2 * l + 1]; q.act = 1;
       q.z_out = z_[l]; q.h_out = h_[l]; q.ldc = H_;
       if (keep_t && l < kLayers - 1) { q.ht_out = ht_[l]; q.ldt = B; }
+      if (keep_t && l == kLayers - 1) {
+        q.w4 = w_ + offs_[6];
+        q.ypart = ypart_;
+        q.ypart_done = &ypart_ready_;
+      }
       EMT_HIPM(mlp_gemm(q, s_));
       in = h_[l];
       K = H_;
     }
-    EMT_HIPM(mlp_gemv_out(h_[2], w_ + offs_[6], w_ + offs_[7], y_, B, H_, s_));
-    EMT_HIPM(mlp_loss(y_, t_, dy_, sq_, B, s_));
+    if (!ypart_ready_)
+      EMT_HIPM(mlp_gemv_out(h_[2], w_ + offs_[6], w_ + offs_[7], y_, B, H_, s_,
+                            keep_t ? t_ : nullptr, dy_, sq_));
   }
 
-  void backward(int B) {
-    // Output layer: dz3 = dy w4 gelu'(z3); its gradients dw4 = dy^T h3 and
-    // db4 = sum dy are column sums, taken with the hidden biases' at the end.
-    EMT_HIPM(mlp_out_back(dy_, w_ + offs_[6], z_[2], dz_[2], dzt_[2], B, H_, s_));
+  // dl (training): the step's loss slots; the slice sums then also store
+  // the loss and advance the clock.
+  void backward(int B, float *dl = nullptr) {
+    // Column-sum partials of the bias gradients (summed with the split-K
+    // slices at the end): the output layer's kernel gives db3, dw4 and db4
+    // per 64-row block, the input-gradient products db2 and db1 per 32-row
+    // block.
+    const int nb64 = (B + 63) / 64, nb32 = (B + 31) / 32;
+    const size_t region = (size_t)nb32 * H_;
+    float *p_db3 = cwork_, *p_dw4 = cwork_ + region, *p_db4 = cwork_ + 2 * region;
+    float *p_db[kLayers - 1] = {cwork_ + 3 * region, cwork_ + 4 * region};
+    // Output layer: dz3 = dy w4 gelu'(z3).
+    OutLoss ol{};
+    if (ypart_ready_) ol = OutLoss{ypart_, (H_ + 63) / 64, w_ + offs_[7], t_, y_, sq_};
+    EMT_HIPM(mlp_out_back(dy_, w_ + offs_[6], z_[2], h_[2], dz_[2], dzt_[2], B, H_, p_db3, p_dw4,
+                          p_db4, ol, s_));
     SliceJobs slices{};
     int nslices = 0;
     for (int l = kLayers - 1; l >= 0; --l) {
@@ -318,40 +416,28 @@ class Mlp {
       if (split > 1)
         slices.job[nslices++] = SliceJob{q.work, (int64_t)H_ * K, split, q.f_out, (int64_t)H_ * K};
       if (l > 0) {
-        // dz_{l-1} = (dz_l W_l) * gelu'(z_{l-1}), W_l from its transposed copy
+        // dz_{l-1} = (dz_l W_l) * gelu'(z_{l-1}), W_l from its transposed
+        // copy; the epilogue also sums dz_{l-1}'s columns (db_{l-1}).
         MlpGemm r{};
         r.A = dz_[l]; r.sam = H_; r.sak = 1;
         r.B = wbt_[l]; r.sbk = 1; r.sbn = H_;
         r.M = B; r.N = H_; r.K = H_;
         r.epi = 1; r.z_in = z_[l - 1]; r.h_out = dz_[l - 1]; r.ldc = H_;
         r.ht_out = dzt_[l - 1]; r.ldt = B;
+        r.colpart = p_db[l - 1];
         EMT_HIPM(mlp_gemm(r, s_));
       }
     }
-    if (nslices) EMT_HIPM(mlp_sum_slices_multi(slices, nslices, s_));
-    // Column sums: dw4, db4 and the three hidden biases.
-    const size_t cw = (size_t)((B + 127) / 128) * H_;
-    ColsumJobs cj{};
-    cj.job[0] = ColsumJob{h_[2], dy_, g_ + offs_[6], cwork_, H_};
-    cj.job[1] = ColsumJob{ones_, dy_, g_ + offs_[7], cwork_ + cw, 1};
-    for (int l = 0; l < kLayers; ++l)
-      cj.job[2 + l] = ColsumJob{dz_[l], nullptr, g_ + offs_[2 * l + 1], cwork_ + (2 + l) * cw, H_};
-    EMT_HIPM(mlp_colsum_multi(cj, 2 + kLayers, B, s_));
+    slices.job[nslices++] = SliceJob{p_db3, H_, nb64, g_ + offs_[5], H_};
+    slices.job[nslices++] = SliceJob{p_dw4, H_, nb64, g_ + offs_[6], H_};
+    slices.job[nslices++] = SliceJob{p_db4, 1, nb64, g_ + offs_[7], 1};
+    for (int l = 0; l < kLayers - 1; ++l)
+      slices.job[nslices++] = SliceJob{p_db[l], H_, nb32, g_ + offs_[2 * l + 1], H_};
+    if (dl) slices.loss = LossTick{sq_, B, dl, clock_, kB1, kB2};
+    EMT_HIPM(mlp_sum_slices_multi(slices, nslices, s_));
   }
 
-  void adam(float lr) {
-    ++step_;
-    AdamBf16 seg{};
-    for (int l = 0; l < kLayers; ++l) {
-      seg.wb[l] = wb_[l];
-      seg.wbt[l] = wbt_[l];
-      seg.off[l] = (int)offs_[2 * l];
-      seg.n[l] = (int)sizes_[2 * l];
-      seg.cols[l] = l == 0 ? kin_ : H_;
-    }
-    EMT_HIPM(mlp_adam_all(w_, m_, v_, g_, seg, (int)total_, lr, 0.9f, 0.999f, 1e-8f, step_, s_));
-  }
-
+  static constexpr float kB1 = 0.9f, kB2 = 0.999f;
   int in_, kin_, H_, device_;
   hipStream_t s_ = nullptr;
   std::vector<int64_t> sizes_, offs_;
@@ -359,15 +445,19 @@ class Mlp {
   float *w_ = nullptr, *m_ = nullptr, *v_ = nullptr, *g_ = nullptr;
   uint16_t *wb_[kLayers] = {}, *wbt_[kLayers] = {};
   int bmax_ = 0;
-  uint16_t *x_ = nullptr, *z_[kLayers] = {}, *h_[kLayers] = {}, *dz_[kLayers] = {}, *ones_ = nullptr;
+  uint16_t *x_ = nullptr, *z_[kLayers] = {}, *h_[kLayers] = {}, *dz_[kLayers] = {};
   uint16_t *xt_ = nullptr, *ht_[kLayers] = {}, *dzt_[kLayers] = {};
   float *y_ = nullptr, *t_ = nullptr, *dy_ = nullptr, *sq_ = nullptr, *xs_ = nullptr;
+  float *ypart_ = nullptr;  // output-layer partial sums, (H / 64) x B
+  int ypart_ready_ = 0;
   float *cwork_ = nullptr, *kwork_ = nullptr;
   int64_t kwork_floats_ = 0;
   uint16_t *data_x_ = nullptr;
   float *data_t_ = nullptr;
   int n_data_ = 0;
-  int step_ = 0;
+  int step_ = 0;          // Adam steps taken
+  int graph_steps_ = 0;
+  TrainClock *clock_ = nullptr;
   std::vector<void *> allocs_;
 };
 
@@ -425,6 +515,64 @@ int emt6ro_mlp_train(emt6ro_mlp *m, int32_t steps, int32_t batch, float lr, uint
   return mlp_guarded([&] {
     std::vector<float> l = m->m->train(steps, batch, lr, seed);
     if (losses) std::memcpy(losses, l.data(), l.size() * 4);
+  });
+}
+
+int emt6ro_mlp_train_many(emt6ro_mlp *const *nets, int32_t count, int32_t steps, int32_t batch,
+                          float lr, const uint64_t *seeds, float *losses) {
+  return mlp_guarded([&] {
+    if (!nets || !seeds || count < 1)
+      throw emt6ro::Error("Check failed: nets, seeds and count >= 1", EMT6RO_ERR_ARG);
+    for (int i = 0; i < count; ++i) {
+      if (!nets[i]) throw emt6ro::Error("Check failed: null network", EMT6RO_ERR_ARG);
+      for (int j = 0; j < i; ++j)
+        if (nets[j] == nets[i]) throw emt6ro::Error("a network is listed twice", EMT6RO_ERR_ARG);
+      if (nets[i]->m->device() != nets[0]->m->device())
+        throw emt6ro::Error("the networks are on different devices", EMT6RO_ERR_ARG);
+    }
+    std::vector<float *> dl(count);
+    for (int i = 0; i < count; ++i) dl[i] = nets[i]->m->train_begin(steps, batch);
+    if (steps == 0) return;
+    // Step k of every network before step k + 1 of any: each network's
+    // kernels stay in order on its own stream, and the streams overlap.  With
+    // graphs (the first network's setting), g steps of each network are
+    // captured once and the graphs are launched in turn.
+    int done = 0;
+    const int g = nets[0]->m->graph_steps();
+    if (g > 0 && steps >= g) {
+      std::vector<hipGraphExec_t> ex;
+      hipError_t e = hipSuccess;
+      try {
+        for (int i = 0; i < count; ++i)
+          ex.push_back(nets[i]->m->capture_steps(g, batch, lr, seeds[i], dl[i]));
+      } catch (...) {
+        for (hipGraphExec_t x : ex) (void)hipGraphExecDestroy(x);
+        throw;
+      }
+      for (; e == hipSuccess && done + g <= steps; done += g)
+        for (int i = 0; i < count && e == hipSuccess; ++i)
+          e = hipGraphLaunch(ex[i], nets[i]->m->stream());
+      for (int i = 0; i < count; ++i) {
+        const hipError_t e2 = hipStreamSynchronize(nets[i]->m->stream());
+        if (e == hipSuccess) e = e2;
+      }
+      for (hipGraphExec_t x : ex) (void)hipGraphExecDestroy(x);
+      if (e != hipSuccess)
+        throw emt6ro::Error(emt6ro::cat("HIP error ", hipGetErrorString(e), " in train_many"), -3);
+    }
+    for (; done < steps; ++done)
+      for (int i = 0; i < count; ++i) nets[i]->m->enqueue_step(batch, lr, seeds[i], dl[i]);
+    for (int i = 0; i < count; ++i) {
+      std::vector<float> l = nets[i]->m->train_end(dl[i], steps);
+      if (losses) std::memcpy(losses + (size_t)i * steps, l.data(), l.size() * 4);
+    }
+  });
+}
+
+int emt6ro_mlp_set_train_graph(emt6ro_mlp *m, int32_t steps_per_graph) {
+  return mlp_guarded([&] {
+    if (!m) throw emt6ro::Error("Check failed: m; null network", EMT6RO_ERR_ARG);
+    m->m->set_graph_steps(steps_per_graph);
   });
 }
 

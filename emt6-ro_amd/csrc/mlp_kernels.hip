@@ -75,6 +75,10 @@ struct GemmArgs {
   int64_t split_stride;        // kEpiStoreF32 with split K: floats between slices
   uint16_t *ht_out;            // optional transposed copy of h_out: ht_out[n * ldt + m]
   int64_t ldt;
+  float *colpart;              // kEpiBackGelu: column sums of h_out per 32-row block
+  int stage;                   // 64 x 64 tiles: stage epilogues 0 and 1 through LDS
+  const float *w4;             // staged kEpiForward: row dot products of the tile with w4
+  float *ypart;                //   ypart[(n0 / 64) * M + m]
 };
 
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
@@ -120,6 +124,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
   }
   // acc[j]: row 8 (j / 4) + 4 (lane / 32) + j % 4, column lane % 32.
   const int n = n0 + wn + (lane & 31);
+  float cs = 0.f;  // column sum of the stored h_out values (kEpiBackGelu)
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     const int m = m0 + wm + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
@@ -137,15 +142,22 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
       const uint16_t hb = to_bf16(v * gelu_grad(from_bf16(g.z_in[o])));
       g.h_out[o] = hb;
       if (g.ht_out) g.ht_out[(int64_t)n * g.ldt + m] = hb;
+      cs += from_bf16(hb);
     } else {
       g.f_out[(int64_t)blockIdx.z * g.split_stride + o] = v;
     }
   }
+  // Per 32-row block column sums, as gemm128_kernel writes them.
+  if (g.colpart) {
+    const float c = cs + __shfl_xor(cs, 32, 64);
+    if (lane < 32 && n < g.N && m0 + wm < g.M)
+      g.colpart[(int64_t)((m0 + wm) >> 5) * g.ldc + n] = c;
+  }
 }
 
-// Epilogue of one accumulator element (row m, column n, value v); returns
-// the bf16 value stored to h_out (epilogues 0 and 1).
-__device__ __forceinline__ uint16_t gemm_store(const GemmArgs &g, int m, int n, float v) {
+// Epilogue of one accumulator element (row m, column n, value v) of split-K
+// slice bz; returns the bf16 value stored to h_out (epilogues 0 and 1).
+__device__ __forceinline__ uint16_t gemm_store(const GemmArgs &g, int m, int n, float v, int bz) {
   if (m >= g.M || n >= g.N) return 0;
   const int64_t o = (int64_t)m * g.ldc + n;
   if (g.epi == kEpiForward) {
@@ -160,9 +172,38 @@ __device__ __forceinline__ uint16_t gemm_store(const GemmArgs &g, int m, int n, 
     g.h_out[o] = hb;
     return hb;
   } else {
-    g.f_out[(int64_t)blockIdx.z * g.split_stride + o] = v;
+    g.f_out[(int64_t)bz * g.split_stride + o] = v;
   }
   return 0;
+}
+
+// The values of one element for epilogues 0 and 1 (as gemm_store computes
+// them): zb the pre-activation (epilogue 0), hb the stored output.
+__device__ __forceinline__ void epi_values(const GemmArgs &g, int m, int n, float v, uint16_t &zb,
+                                           uint16_t &hb) {
+  if (g.epi == kEpiForward) {
+    zb = to_bf16(v + g.bias[n]);
+    hb = g.act ? to_bf16(gelu(from_bf16(zb))) : zb;
+  } else {
+    zb = 0;
+    hb = to_bf16(v * gelu_grad(from_bf16(g.z_in[(int64_t)m * g.ldc + n])));
+  }
+}
+
+// Workgroups are dealt to the GPU's 8 XCDs (each with its own L2) in turn by
+// linear id.  The tile of a workgroup is chosen so that each XCD takes a
+// contiguous run of tiles, x fastest, then y, then z: the rows of A it reads
+// (a forward or input-gradient product: 8 row tiles) or its split-K slice (a
+// weight-gradient product) then stay in that XCD's L2, where the plain order
+// would give every XCD one column tile and make each read all of A.
+constexpr int kXcds = 8;
+__device__ __forceinline__ void xcd_tile(int &bx, int &by, int &bz) {
+  const int nx = gridDim.x, ny = gridDim.y, total = nx * ny * (int)gridDim.z;
+  int l = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+  if (total % kXcds == 0) l = (l % kXcds) * (total / kXcds) + l / kXcds;
+  bx = l % nx;
+  by = (l / nx) % ny;
+  bz = l / (nx * ny);
 }
 
 // The same product on TM x TN tiles (128 x 128, 128 x 64 or 64 x 64) with 16-byte
@@ -219,9 +260,11 @@ __global__ __launch_bounds__(256) void gemm128_kernel(GemmArgs g) {
   __shared__ __align__(16) uint16_t As[TM][KK + 8];
   __shared__ __align__(16) uint16_t Bs[TN][KK + 8];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  int bx, by, bz;
+  xcd_tile(bx, by, bz);
+  const int m0 = by * TM, n0 = bx * TN;
   const int wm = (wave >> 1) * (TM / 2), wn = (wave & 1) * (TN / 2);
-  const int kb = blockIdx.z * g.k_chunk;
+  const int kb = bz * g.k_chunk;
   const int ke = min(g.K, kb + g.k_chunk);
   f32x16 acc[XB][YB];
 #pragma unroll
@@ -230,6 +273,22 @@ __global__ __launch_bounds__(256) void gemm128_kernel(GemmArgs g) {
     for (int b = 0; b < YB; ++b)
 #pragma unroll
       for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.f;
+  // Output-layer partial sums: this tile's 64 entries of w4, fetched now.
+  __shared__ float w4s[64];
+  float w4v = 0.f;
+  if (XB == 1 && YB == 1 && g.ypart && tid < 64 && n0 + tid < g.N) w4v = g.w4[n0 + tid];
+  // Staged input-gradient epilogue: its 64 x 64 tile of z_in is fetched now,
+  // 16 bytes a load, so that its latency hides behind the products.
+  const bool zpre = XB == 1 && YB == 1 && g.stage && g.epi == kEpiBackGelu;
+  uint4 zr[2] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
+  if (zpre) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int e = tid + 256 * p, m = m0 + (e >> 3), nc = n0 + (e & 7) * 8;
+      if (m < g.M && nc < g.N)
+        zr[p] = *reinterpret_cast<const uint4 *>(g.z_in + (int64_t)m * g.ldc + nc);
+    }
+  }
   // A(m, k) = A[m sam + k sak]; B(k, n) = B[k sbk + n sbn].
   uint4 ra[NA], rb[NB];
   load_chunks<AK, TM, KK>(g.A, g.sam, g.sak, m0, kb, g.M, ke, ra);
@@ -261,8 +320,92 @@ __global__ __launch_bounds__(256) void gemm128_kernel(GemmArgs g) {
     }
   }
   // acc[x][y][j]: row 8 (j / 4) + 4 (lane / 32) + j % 4 of block x, column
-  // lane % 32 of block y; the 4 rows of each j / 4 go to the transposed copy
-  // as one 8-byte store.
+  // lane % 32 of block y.
+  if (XB == 1 && YB == 1 && g.stage) {
+    // 64 x 64 tiles, epilogues 0 and 1: the bf16 results go through LDS
+    // (the operand tiles, free after the last product) so that h, z and the
+    // transposed copy leave as 16-byte stores of 8 consecutive elements.
+    __syncthreads();
+    uint16_t(*Hs)[KK + 8] = As;
+    uint16_t(*Zs)[KK + 8] = Bs;
+    if (zpre) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const int e = tid + 256 * p;
+        *reinterpret_cast<uint4 *>(&Zs[e >> 3][(e & 7) * 8]) = zr[p];
+      }
+      __syncthreads();
+    }
+    const int nl = wn + (lane & 31), n = n0 + nl;
+    float cs0 = 0.f;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int ml = wm + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3), m = m0 + ml;
+      uint16_t zb = 0, hb = 0;
+      if (m < g.M && n < g.N) {
+        if (zpre)  // epi_values' input-gradient value, z from the fetched tile
+          hb = to_bf16(acc[0][0][j] * gelu_grad(from_bf16(Zs[ml][nl])));
+        else
+          epi_values(g, m, n, acc[0][0][j], zb, hb);
+      }
+      Hs[ml][nl] = hb;
+      if (!zpre) Zs[ml][nl] = zb;
+      cs0 += from_bf16(hb);
+    }
+    if (g.ypart && tid < 64) w4s[tid] = w4v;
+    __syncthreads();
+    if (g.ypart) {
+      // Row r of the tile against w4: four threads take 16 columns each,
+      // added in a fixed order ((q0 + q1) + (q2 + q3)).
+      const int r = tid >> 2, qq = tid & 3;
+      float p = 0.f;
+#pragma unroll
+      for (int c = 16 * qq; c < 16 * qq + 16; ++c)
+        if (n0 + c < g.N) p += from_bf16(Hs[r][c]) * w4s[c];
+      p += __shfl_xor(p, 1, 64);
+      p += __shfl_xor(p, 2, 64);
+      if (qq == 0 && m0 + r < g.M) g.ypart[(int64_t)bx * g.M + m0 + r] = p;
+    }
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int e = tid + 256 * p, r = e >> 3, cj = (e & 7) * 8;
+      const int m = m0 + r, nc = n0 + cj;
+      if (m < g.M && nc < g.N) {
+        const int64_t o = (int64_t)m * g.ldc + nc;
+        *reinterpret_cast<uint4 *>(g.h_out + o) = *reinterpret_cast<const uint4 *>(&Hs[r][cj]);
+        if (g.z_out)
+          *reinterpret_cast<uint4 *>(g.z_out + o) = *reinterpret_cast<const uint4 *>(&Zs[r][cj]);
+      }
+    }
+    if (g.ht_out) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const int e = tid + 256 * p, c = e >> 3, rj = (e & 7) * 8;
+        const int nc = n0 + c, m = m0 + rj;
+        if (nc < g.N && m < g.M) {
+          uint32_t w[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            w[k] = (uint32_t)Hs[rj + 2 * k][c] | ((uint32_t)Hs[rj + 2 * k + 1][c] << 16);
+          *reinterpret_cast<uint4 *>(g.ht_out + (int64_t)nc * g.ldt + m) =
+              make_uint4(w[0], w[1], w[2], w[3]);
+        }
+      }
+    }
+    if (g.colpart) {
+      const float c = cs0 + __shfl_xor(cs0, 32, 64);
+      if (lane < 32 && n < g.N && m0 + wm < g.M)
+        g.colpart[(int64_t)((m0 + wm) >> 5) * g.ldc + n] = c;
+    }
+    return;
+  }
+  // Otherwise element by element; the 4 rows of each j / 4 go to the
+  // transposed copy as one 8-byte store.
+  float cs[XB][YB];
+#pragma unroll
+  for (int x = 0; x < XB; ++x)
+#pragma unroll
+    for (int y = 0; y < YB; ++y) cs[x][y] = 0.f;
 #pragma unroll
   for (int x = 0; x < XB; ++x)
 #pragma unroll
@@ -273,36 +416,50 @@ __global__ __launch_bounds__(256) void gemm128_kernel(GemmArgs g) {
         const int n = n0 + wn + y * 32 + (lane & 31);
         uint16_t hv[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) hv[i] = gemm_store(g, mb + i, n, acc[x][y][4 * q + i]);
+        for (int i = 0; i < 4; ++i) hv[i] = gemm_store(g, mb + i, n, acc[x][y][4 * q + i], bz);
         if (g.ht_out && mb < g.M && n < g.N) {
           const uint2 w2 = make_uint2((uint32_t)hv[0] | ((uint32_t)hv[1] << 16),
                                       (uint32_t)hv[2] | ((uint32_t)hv[3] << 16));
           *reinterpret_cast<uint2 *>(g.ht_out + (int64_t)n * g.ldt + mb) = w2;
         }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) cs[x][y] += from_bf16(hv[i]);
       }
+  // Column sums of the stored bf16 values over each 32-row block (rows past
+  // M stored nothing and add 0): lanes l and l + 32 hold the same column.
+  if (g.colpart) {
+#pragma unroll
+    for (int x = 0; x < XB; ++x)
+#pragma unroll
+      for (int y = 0; y < YB; ++y) {
+        const float c = cs[x][y] + __shfl_xor(cs[x][y], 32, 64);
+        const int n = n0 + wn + y * 32 + (lane & 31), mblk = (m0 + wm + x * 32) >> 5;
+        if (lane < 32 && n < g.N && m0 + wm + x * 32 < g.M) g.colpart[(int64_t)mblk * g.ldc + n] = c;
+      }
+  }
 }
 
-// y[b] = sum_k h[b, k] w[k] + bias   (one wave per row)
+// y[b] = sum_k h[b, k] w[k] + bias   (one wave per row); with targets t,
+// also the loss gradient dy = (y - t) / B and the squared error per row.
 __global__ __launch_bounds__(256) void gemv_out_kernel(const uint16_t *h, const float *w,
-                                                       const float *bias, float *y, int B, int K) {
+                                                       const float *bias, float *y, int B, int K,
+                                                       const float *t, float *dy, float *sq) {
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (row >= B) return;
   float s = 0.f;
   for (int k = lane; k < K; k += 64) s += from_bf16(h[(int64_t)row * K + k]) * w[k];
   for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
-  if (lane == 0) y[row] = s + bias[0];
+  if (lane == 0) {
+    const float yv = s + bias[0];
+    y[row] = yv;
+    if (t) {
+      const float d = yv - t[row];
+      dy[row] = d / (float)B;
+      sq[row] = d * d;
+    }
+  }
 }
 
-// Loss gradient dy = (y - t) / B and per-row squared error.
-__global__ void loss_kernel(const float *y, const float *t, float *dy, float *sq, int B) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= B) return;
-  const float d = y[i] - t[i];
-  dy[i] = d / (float)B;
-  sq[i] = d * d;
-}
-
-// dz[b, k] = dy[b] w[k] gelu'(z[b, k])  (bf16)
 // A 64 x 64 tile of a row-major bf16 result, written to out (row-major,
 // ld cols) and through LDS to out_t (its transpose, ld rows), both with
 // coalesced stores.  value(r, c) gives the element; 256 threads.
@@ -330,12 +487,84 @@ __device__ __forceinline__ void tile_store_both(int r0, int c0, int rows, int co
 }
 
 // dz = dy w4 gelu'(z3) (and its transpose), one 64 x 64 tile per block.
-__global__ __launch_bounds__(256) void out_back_kernel(const float *dy, const float *w,
-                                                       const uint16_t *z, uint16_t *dz,
-                                                       uint16_t *dzt, int B, int K) {
-  tile_store_both(blockIdx.y * 64, blockIdx.x * 64, B, K, dz, dzt, [&](int b, int k) {
-    return to_bf16(dy[b] * w[k] * gelu_grad(from_bf16(z[(int64_t)b * K + k])));
-  });
+// The output layer's backward pass, one 64 x 64 tile per block: dz = dy w
+// gelu'(z) (bf16), row-major and (through LDS) transposed, and the sums over
+// the tile's 64 rows of dz, of dy h and of dy that make the gradients of
+// b3, w4 and b4.  Thread t takes column t % 64 and rows t / 64 + 4 j; its
+// four partial sums per column are added in order.
+__global__ __launch_bounds__(256) void out_back_kernel(float *dy, const float *w,
+                                                       const uint16_t *z, const uint16_t *h,
+                                                       uint16_t *dz, uint16_t *dzt, int B, int K,
+                                                       float *pdz, float *pdw, float *pdb,
+                                                       OutLoss out) {
+  __shared__ uint16_t tile[64][66];
+  __shared__ float red[3][4][64];
+  __shared__ float dys[64];
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64, tid = threadIdx.x;
+  const int c = tid & 63, q = tid >> 6;
+  if (out.ypart) {
+    // The output and the loss terms of the block's 64 rows (the partial dot
+    // products added in column-tile order); column block 0 stores them.
+    if (tid < 64) {
+      const int b = r0 + tid;
+      float dv = 0.f;
+      if (b < B) {
+        float yv = 0.f;
+        for (int t0 = 0; t0 < out.nyp; t0 += 8) {
+          float v[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
+            v[k] = t0 + k < out.nyp ? out.ypart[(int64_t)(t0 + k) * B + b] : 0.f;
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
+            if (t0 + k < out.nyp) yv += v[k];
+        }
+        yv += out.b4[0];
+        const float d = yv - out.t[b];
+        dv = d / (float)B;
+        if (blockIdx.x == 0) {
+          out.y[b] = yv;
+          dy[b] = dv;
+          out.sq[b] = d * d;
+        }
+      }
+      dys[tid] = dv;
+    }
+    __syncthreads();
+  }
+  const bool colok = c0 + c < K;
+  float sdz = 0.f, sdw = 0.f, sdb = 0.f;
+#pragma unroll 4
+  for (int r = q; r < 64; r += 4) {
+    const int b = r0 + r;
+    if (b < B && colok) {
+      const int64_t o = (int64_t)b * K + c0 + c;
+      const float d = out.ypart ? dys[r] : dy[b];
+      const uint16_t v = to_bf16(d * w[c0 + c] * gelu_grad(from_bf16(z[o])));
+      dz[o] = v;
+      tile[r][c] = v;
+      sdz += from_bf16(v);
+      sdw += d * from_bf16(h[o]);
+      sdb += d;
+    }
+  }
+  red[0][q][c] = sdz;
+  red[1][q][c] = sdw;
+  red[2][q][c] = sdb;
+  __syncthreads();
+#pragma unroll 4
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int cc = e >> 6, rr = e & 63;
+    if (r0 + rr < B && c0 + cc < K) dzt[(int64_t)(c0 + cc) * B + r0 + rr] = tile[rr][cc];
+  }
+  if (q == 0 && colok) {
+    pdz[(int64_t)blockIdx.y * K + c0 + c] =
+        ((red[0][0][c] + red[0][1][c]) + red[0][2][c]) + red[0][3][c];
+    pdw[(int64_t)blockIdx.y * K + c0 + c] =
+        ((red[1][0][c] + red[1][1][c]) + red[1][2][c]) + red[1][3][c];
+  }
+  if (blockIdx.x == 0 && tid == 0)
+    pdb[blockIdx.y] = ((red[2][0][0] + red[2][1][0]) + red[2][2][0]) + red[2][3][0];
 }
 
 // yt[c * rows + r] = x[r * cols + c] (bf16).
@@ -356,52 +585,58 @@ __global__ void sum_slices_kernel(const float *part, int64_t stride, int nsplit,
   out[i] = s;
 }
 
-// Column sums, stage 1: block (bx, by) sums rows [by * kColRows, ...) of
-// columns [bx * 64, ...): out[k] = sum_b x[b, k] (x bf16), or sum_b dy[b] x[b, k].
-constexpr int kColRows = 128;
-__global__ __launch_bounds__(256) void colsum_part_kernel(const uint16_t *x, const float *rowscale,
-                                                          float *part, int B, int K) {
-  __shared__ float red[4][64];
-  const int k = blockIdx.x * 64 + (threadIdx.x & 63), r0 = threadIdx.x >> 6;
-  const int b0 = blockIdx.y * kColRows, b1 = min(B, b0 + kColRows);
-  float s = 0.f;
-  if (k < K)
-    for (int b = b0 + r0; b < b1; b += 4) {
-      const float v = from_bf16(x[(int64_t)b * K + k]);
-      s += rowscale ? rowscale[b] * v : v;
-    }
-  red[r0][threadIdx.x & 63] = s;
-  __syncthreads();
-  if (r0 == 0 && k < K)
-    part[(int64_t)blockIdx.y * K + k] =
-        ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
-}
-
-// Batched forms of the two kernels above for the training step: blockIdx.z
-// (column sums) or blockIdx.y (slice sums) picks one of up to kMaxJobs jobs,
-// each summed exactly as the single-job kernels do.
-__global__ __launch_bounds__(256) void colsum_part_multi_kernel(ColsumJobs j, int B) {
-  const ColsumJob &c = j.job[blockIdx.z];
-  if ((int)blockIdx.x * 64 >= c.K) return;
-  __shared__ float red[4][64];
-  const int K = c.K;
-  const int k = blockIdx.x * 64 + (threadIdx.x & 63), r0 = threadIdx.x >> 6;
-  const int b0 = blockIdx.y * kColRows, b1 = min(B, b0 + kColRows);
-  float sum = 0.f;
-  if (k < K)
-    for (int b = b0 + r0; b < b1; b += 4) {
-      const float v = from_bf16(c.x[(int64_t)b * K + k]);
-      sum += c.rowscale ? c.rowscale[b] * v : v;
-    }
-  red[r0][threadIdx.x & 63] = sum;
-  __syncthreads();
-  if (r0 == 0 && k < K)
-    c.part[(int64_t)blockIdx.y * K + k] =
-        ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
-}
-
+// Slice sums of up to kMaxJobs jobs (blockIdx.y picks one), each in the
+// fixed order of sum_slices_kernel: the split-K slices of the weight
+// gradients and the per-row-block column sums of the bias gradients.
+constexpr int kWideSlices = 32;
 __global__ void sum_slices_multi_kernel(SliceJobs j) {
+  if (blockIdx.y == gridDim.y - 1 && j.loss.sq) {
+    // The loss job: block 0 adds sq in a fixed order, stores the mean in
+    // this step's slot and advances the clock (the products rounded on their
+    // own, as the host computes them).
+    if (blockIdx.x != 0) return;
+    const LossTick &L = j.loss;
+    __shared__ float wsum[4];
+    // Thread t adds rows t, t + 256, ..., eight loads in flight at a time;
+    // then a butterfly per wave and the four waves in order.
+    float sum = 0.f;
+    for (int b0 = 0; b0 < L.B; b0 += 8 * 256) {
+      float v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int b = b0 + k * 256 + (int)threadIdx.x;
+        v[k] = b < L.B ? L.sq[b] : 0.f;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) sum += v[k];
+    }
+    for (int m = 32; m >= 1; m >>= 1) sum += __shfl_xor(sum, m, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      sum = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+      const int it = L.clk->it;
+      L.out[it] = sum / (float)L.B;
+      L.clk->it = it + 1;
+      L.clk->p1 = __fmul_rn(L.clk->p1, L.b1);
+      L.clk->p2 = __fmul_rn(L.clk->p2, L.b2);
+    }
+    return;
+  }
   const SliceJob &c = j.job[blockIdx.y];
+  if (c.nsplit >= kWideSlices) {
+    // Many slices (the bias gradients' row blocks): one wave per output;
+    // lane l adds slices l, l + 64, ... in order, then the lanes are added
+    // by a fixed butterfly.
+    const int64_t i = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= c.n) return;  // the whole wave
+    float sum = 0.f;
+    for (int z = lane; z < c.nsplit; z += 64) sum += c.part[(int64_t)z * c.stride + i];
+    for (int m = 32; m >= 1; m >>= 1) sum += __shfl_xor(sum, m, 64);
+    if (lane == 0) c.out[i] = sum;
+    return;
+  }
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c.n) return;
   float sum = c.part[i];
@@ -409,28 +644,16 @@ __global__ void sum_slices_multi_kernel(SliceJobs j) {
   c.out[i] = sum;
 }
 
-// Adam (Kingma & Ba, 2015) with bias correction; refreshes the bf16 copy.
-__global__ void adam_kernel(float *w, float *m, float *v, const float *g, uint16_t *wb, int n,
-                            float lr, float b1, float b2, float eps, float c1, float c2) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i];
-  const float mi = b1 * m[i] + (1.f - b1) * gi;
-  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-  m[i] = mi;
-  v[i] = vi;
-  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-  w[i] = wi;
-  if (wb) wb[i] = to_bf16(wi);
-}
-
-// Adam over the whole flat parameter vector in one launch; elements of the
-// kAdamBf16 segments [off, off + n) also refresh their bf16 copies.
+// Adam (Kingma & Ba, 2015) with bias correction over the whole flat
+// parameter vector in one launch; elements of the kAdamBf16 segments
+// [off, off + n) also refresh their bf16 copies.  The clock holds b1^t and
+// b2^t for this step t: c1 = 1 - b1^t, c2 = 1 - b2^t.
 __global__ void adam_all_kernel(float *w, float *m, float *v, const float *g, AdamBf16 seg,
-                                int n, float lr, float b1, float b2, float eps, float c1,
-                                float c2) {
+                                int n, float lr, float b1, float b2, float eps,
+                                const TrainClock *clk) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const float c1 = 1.f - clk->p1, c2 = 1.f - clk->p2;
   const float gi = g[i];
   const float mi = b1 * m[i] + (1.f - b1) * gi;
   const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
@@ -469,10 +692,11 @@ __device__ __forceinline__ int32_t pick_row(uint64_t seed, uint32_t step, uint32
 // are drawn on the device, so a training step needs no host round trip.
 // One 64-row x 64-column tile of the minibatch (and its transpose).
 __global__ __launch_bounds__(256) void gather_kernel(const uint16_t *X, const float *t,
-                                                     uint64_t seed, uint32_t step, int n,
+                                                     uint64_t seed, const TrainClock *clk, int n,
                                                      uint16_t *Xb, uint16_t *Xbt, float *tb,
                                                      int B, int ld) {
   const int b0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const uint32_t step = (uint32_t)clk->it;
   __shared__ int32_t src[64];
   if (threadIdx.x < 64 && b0 + (int)threadIdx.x < B) {
     const int b = b0 + threadIdx.x;
@@ -485,19 +709,6 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint16_t *X, const fl
   });
 }
 
-// out[0] = mean of sq[0..B) (one workgroup).
-__global__ __launch_bounds__(256) void mean_kernel(const float *sq, int B, float *out) {
-  __shared__ float part[256];
-  float s = 0.f;
-  for (int i = threadIdx.x; i < B; i += 256) s += sq[i];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w >= 1; w >>= 1) {
-    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[0] = part[0] / (float)B;
-}
 
 }  // namespace mlp
 
@@ -529,6 +740,13 @@ hipError_t mlp_gemm(const MlpGemm &q, hipStream_t s) {
   g.epi = q.epi; g.bias = q.bias; g.act = q.act;
   g.z_out = q.z_out; g.h_out = q.h_out; g.z_in = q.z_in; g.f_out = q.f_out; g.ldc = q.ldc;
   g.ht_out = q.ht_out; g.ldt = q.ldt;
+  g.colpart = q.epi == mlp::kEpiBackGelu ? q.colpart : nullptr;
+  // The staged epilogue stores 8 elements (16 bytes) at a time.
+  auto al16 = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
+  g.stage = q.epi != mlp::kEpiStoreF32 && q.ldc % 8 == 0 && al16(q.h_out) &&
+            (!q.z_out || al16(q.z_out)) && (!q.ht_out || (q.ldt % 8 == 0 && al16(q.ht_out)));
+  g.w4 = q.w4;
+  g.ypart = nullptr;
   const bool a_k = q.sak == 1, b_k = q.sbk == 1;
   const int64_t sa = a_k ? q.sam : q.sak, sb = b_k ? q.sbn : q.sbk;
   const bool wide = q.M % 8 == 0 && q.N % 8 == 0 && q.K % 8 == 0 && sa % 8 == 0 &&
@@ -567,6 +785,10 @@ hipError_t mlp_gemm(const MlpGemm &q, hipStream_t s) {
     g.f_out = q.work;
     g.split_stride = (int64_t)q.M * q.ldc;
   }
+  // The output-layer partial sums need the staged 64 x 64 epilogue.
+  if (q.ypart && q.epi == mlp::kEpiForward && wide && g.stage && tm == 64 && tn == 64 && split == 1)
+    g.ypart = q.ypart;
+  if (q.ypart_done) *q.ypart_done = g.ypart != nullptr;
   if (wide) {
     dim3 grid2((q.N + tn - 1) / tn, (q.M + tm - 1) / tm, split);
     if (tm == 64)
@@ -589,20 +811,17 @@ hipError_t mlp_gemm(const MlpGemm &q, hipStream_t s) {
 }
 
 hipError_t mlp_gemv_out(const uint16_t *h, const float *w, const float *bias, float *y, int B,
-                        int K, hipStream_t s) {
-  hipLaunchKernelGGL(mlp::gemv_out_kernel, dim3(blocks(B, 4)), dim3(256), 0, s, h, w, bias, y, B, K);
+                        int K, hipStream_t s, const float *t, float *dy, float *sq) {
+  hipLaunchKernelGGL(mlp::gemv_out_kernel, dim3(blocks(B, 4)), dim3(256), 0, s, h, w, bias, y, B, K,
+                     t, dy, sq);
   return hipGetLastError();
 }
 
-hipError_t mlp_loss(const float *y, const float *t, float *dy, float *sq, int B, hipStream_t s) {
-  hipLaunchKernelGGL(mlp::loss_kernel, dim3(blocks(B, 256)), dim3(256), 0, s, y, t, dy, sq, B);
-  return hipGetLastError();
-}
-
-hipError_t mlp_out_back(const float *dy, const float *w, const uint16_t *z, uint16_t *dz,
-                        uint16_t *dzt, int B, int K, hipStream_t s) {
+hipError_t mlp_out_back(float *dy, const float *w, const uint16_t *z, const uint16_t *h,
+                        uint16_t *dz, uint16_t *dzt, int B, int K, float *pdz, float *pdw,
+                        float *pdb, const OutLoss &out, hipStream_t s) {
   hipLaunchKernelGGL(mlp::out_back_kernel, dim3(blocks(K, 64), blocks(B, 64)), dim3(256), 0, s, dy,
-                     w, z, dz, dzt, B, K);
+                     w, z, h, dz, dzt, B, K, pdz, pdw, pdb, out);
   return hipGetLastError();
 }
 
@@ -612,59 +831,23 @@ hipError_t mlp_transpose_bf16(const uint16_t *x, int rows, int cols, uint16_t *y
   return hipGetLastError();
 }
 
-hipError_t mlp_colsum(const uint16_t *x, const float *rowscale, float *out, int B, int K,
-                      float *work, hipStream_t s) {
-  const int nb = (B + mlp::kColRows - 1) / mlp::kColRows;
-  hipLaunchKernelGGL(mlp::colsum_part_kernel, dim3(blocks(K, 64), nb), dim3(256), 0, s, x,
-                     rowscale, work, B, K);
-  hipLaunchKernelGGL(mlp::sum_slices_kernel, dim3(blocks(K, 256)), dim3(256), 0, s, work,
-                     (int64_t)K, nb, out, (int64_t)K);
-  return hipGetLastError();
-}
-
-hipError_t mlp_colsum_multi(const ColsumJobs &jobs, int njobs, int B, hipStream_t s) {
-  if (njobs <= 0 || njobs > kMaxJobs) return hipErrorInvalidValue;
-  int kmax = 0;
-  SliceJobs sj{};
-  const int nb = (B + mlp::kColRows - 1) / mlp::kColRows;
-  for (int i = 0; i < njobs; ++i) {
-    kmax = std::max(kmax, jobs.job[i].K);
-    sj.job[i] = SliceJob{jobs.job[i].part, (int64_t)jobs.job[i].K, nb, jobs.job[i].out,
-                         (int64_t)jobs.job[i].K};
-  }
-  hipLaunchKernelGGL(mlp::colsum_part_multi_kernel, dim3(blocks(kmax, 64), nb, njobs), dim3(256), 0,
-                     s, jobs, B);
-  return mlp_sum_slices_multi(sj, njobs, s);
-}
-
 hipError_t mlp_sum_slices_multi(const SliceJobs &jobs, int njobs, hipStream_t s) {
-  if (njobs <= 0 || njobs > kMaxJobs) return hipErrorInvalidValue;
-  int64_t nmax = 0;
-  for (int i = 0; i < njobs; ++i) nmax = std::max(nmax, jobs.job[i].n);
-  hipLaunchKernelGGL(mlp::sum_slices_multi_kernel, dim3(blocks(nmax, 256), njobs), dim3(256), 0, s,
-                     jobs);
+  if (njobs < 0 || njobs > kMaxJobs || (njobs == 0 && !jobs.loss.sq)) return hipErrorInvalidValue;
+  int bx = 1;
+  for (int i = 0; i < njobs; ++i) {
+    const SliceJob &c = jobs.job[i];
+    bx = std::max(bx, c.nsplit >= mlp::kWideSlices ? blocks(c.n, 4) : blocks(c.n, 256));
+  }
+  hipLaunchKernelGGL(mlp::sum_slices_multi_kernel, dim3(bx, njobs + (jobs.loss.sq ? 1 : 0)),
+                     dim3(256), 0, s, jobs);
   return hipGetLastError();
 }
 
 hipError_t mlp_adam_all(float *w, float *m, float *v, const float *g, const AdamBf16 &seg, int n,
-                        float lr, float b1, float b2, float eps, int t, hipStream_t s) {
-  float c1 = 1.f, c2 = 1.f;
-  for (int i = 0; i < t; ++i) { c1 *= b1; c2 *= b2; }
-  c1 = 1.f - c1;
-  c2 = 1.f - c2;
+                        float lr, float b1, float b2, float eps, const TrainClock *clk,
+                        hipStream_t s) {
   hipLaunchKernelGGL(mlp::adam_all_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, w, m, v, g, seg, n,
-                     lr, b1, b2, eps, c1, c2);
-  return hipGetLastError();
-}
-
-hipError_t mlp_adam(float *w, float *m, float *v, const float *g, uint16_t *wb, int n, float lr,
-                    float b1, float b2, float eps, int t, hipStream_t s) {
-  float c1 = 1.f, c2 = 1.f;
-  for (int i = 0; i < t; ++i) { c1 *= b1; c2 *= b2; }
-  c1 = 1.f - c1;
-  c2 = 1.f - c2;
-  hipLaunchKernelGGL(mlp::adam_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, w, m, v, g, wb, n, lr,
-                     b1, b2, eps, c1, c2);
+                     lr, b1, b2, eps, clk);
   return hipGetLastError();
 }
 
@@ -673,17 +856,14 @@ hipError_t mlp_to_bf16(const float *x, uint16_t *y, int64_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t mlp_gather(const uint16_t *X, const float *t, uint64_t seed, uint32_t step, int n,
-                      uint16_t *Xb, uint16_t *Xbt, float *tb, int B, int ld, hipStream_t s) {
+hipError_t mlp_gather(const uint16_t *X, const float *t, uint64_t seed, const TrainClock *clk,
+                      int n, uint16_t *Xb, uint16_t *Xbt, float *tb, int B, int ld,
+                      hipStream_t s) {
   hipLaunchKernelGGL(mlp::gather_kernel, dim3(blocks(ld, 64), blocks(B, 64)), dim3(256), 0, s, X, t,
-                     seed, step, n, Xb, Xbt, tb, B, ld);
+                     seed, clk, n, Xb, Xbt, tb, B, ld);
   return hipGetLastError();
 }
 
-hipError_t mlp_mean(const float *sq, int B, float *out, hipStream_t s) {
-  hipLaunchKernelGGL(mlp::mean_kernel, dim3(1), dim3(256), 0, s, sq, B, out);
-  return hipGetLastError();
-}
 
 // ---------------------------------------------------------------------------
 // Forward pass that allocates nothing (emt6ro_mlp_predict_noalloc): its

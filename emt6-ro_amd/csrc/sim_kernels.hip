@@ -349,13 +349,15 @@ struct DiffSel<4, NT, A, B, RS, PS> {
                                       PairRunDiffuser<NT, RS, PS, A, 2, 4, 8, 1>>::type;
 };
 // Lane-run band heights and policy by code B: 0 3/4/6, 1 4/6/8 (fewest
-// instructions); 2 4/6/8, 3 2/3/4, 4 3/4/6 (lowest bands that fit).
+// instructions); 2 4/6/8, 3 2/3/4, 4 3/4/6 (lowest bands that fit); 5 3/4/5
+// (fewest instructions by the stencil's measured count).
 template <int B> struct LaneRunShape;
 template <> struct LaneRunShape<0> { static constexpr int h1 = 3, h2 = 4, h3 = 6, pol = 0; };
 template <> struct LaneRunShape<1> { static constexpr int h1 = 4, h2 = 6, h3 = 8, pol = 0; };
 template <> struct LaneRunShape<2> { static constexpr int h1 = 4, h2 = 6, h3 = 8, pol = 1; };
 template <> struct LaneRunShape<3> { static constexpr int h1 = 2, h2 = 3, h3 = 4, pol = 1; };
 template <> struct LaneRunShape<4> { static constexpr int h1 = 3, h2 = 4, h3 = 6, pol = 1; };
+template <> struct LaneRunShape<5> { static constexpr int h1 = 3, h2 = 4, h3 = 5, pol = 2; };
 template <int NT, int A, int B, int RS, int PS>
 struct DiffSel<3, NT, A, B, RS, PS> {
   using S = LaneRunShape<B>;
@@ -690,7 +692,8 @@ hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds
   X(40, 3, 512, 3, 2, 2, 88, 2)   \
   X(41, 3, 256, 2, 4, 2, 36, 4)   \
   X(42, 3, 128, 4, 4, 3, 36, 4)   \
-  X(43, 3, 128, 2, 2, 3, 36, 4)
+  X(43, 3, 128, 2, 2, 3, 36, 4)   \
+  X(44, 3, 256, 3, 5, 2, 56, 4)
 
 const VariantInfo *variant_table(int *count) {
   static const VariantInfo table[] = {
@@ -804,13 +807,18 @@ hipError_t launch_order_by_cost(const int32_t *src, int32_t *dst, int n, const S
   return hipGetLastError();
 }
 
-// The window kernels, lane-run diffusion with band heights 4/6/8 (lowest
-// that fits), 2 lane slots, 4 waves per SIMD: row stride 36 with 128 threads
-// (eight workgroups per CU), and row stride 56 with 256 threads (four per
-// CU, for lattices wider than 54 sites, whose whole-lattice kernel holds one
-// workgroup per CU).  (A 44-column window at six 128-thread workgroups per
-// CU ran slower than the 53x53 whole-lattice kernel.)
-using WindowDiff36 = dev::LaneRunDiffuser<128, 36, 3, 2, 4, 6, 8, 1>;
+// The window kernels, lane-run diffusion, 4 waves per SIMD: row stride 36
+// with 128 threads (eight workgroups per CU), band heights 3/4/5 chosen by
+// the stencil's instruction count, 3 lane slots (1.3% faster over the whole
+// 53x53 workload than heights 4/6/8, lowest first, in 2 slots); row stride 56
+// with 256 threads (four per CU, for lattices wider than 54 sites, whose
+// whole-lattice kernel holds one workgroup per CU), heights 4/6/8.  (A
+// 44-column window at six 128-thread workgroups per CU ran slower than the
+// 53x53 whole-lattice kernel.)
+#ifndef EMT6RO_WINDOW36_SHAPE
+#define EMT6RO_WINDOW36_SHAPE 3, 3, 4, 5, 2
+#endif
+using WindowDiff36 = dev::LaneRunDiffuser<128, 36, 3, EMT6RO_WINDOW36_SHAPE>;
 using WindowDiff56 = dev::LaneRunDiffuser<256, 56, 2, 2, 4, 6, 8, 1>;
 // The window kernels take ROIs up to RS - 4 wide (window_origin) and at
 // most kMaxBoxRows - 2 tall: their two rings must fit the ring slots.

@@ -83,6 +83,8 @@ def _load():
         "emt6ro_mlp_set_params": ([P, f32p], C.c_int),
         "emt6ro_mlp_set_data": ([P, f32p, f32p, i32], C.c_int),
         "emt6ro_mlp_train": ([P, i32, i32, C.c_float, u64, f32p], C.c_int),
+        "emt6ro_mlp_set_train_graph": ([P, i32], C.c_int),
+        "emt6ro_mlp_train_many": ([P, i32, i32, i32, C.c_float, P, f32p], C.c_int),
         "emt6ro_mlp_predict": ([P, f32p, i32, f32p], C.c_int),
         "emt6ro_mlp_predict_noalloc": ([P, P, i32, P], C.c_int),
         "emt6ro_mlp_grads": ([P, f32p, f32p, i32, f32p, f32p], C.c_int),

@@ -233,6 +233,11 @@ class MLP:
                                        _fp(losses)))
         return losses[:int(steps)]
 
+    def set_train_graph(self, steps_per_graph):
+        """Steps per replayed HIP graph in train() (0: launch each kernel
+        directly); the results do not change."""
+        check_mlp(lib.emt6ro_mlp_set_train_graph(self._h, int(steps_per_graph)))
+
     def predict(self, X):
         x = self._x(X)
         y = np.zeros(x.shape[0], np.float32)
@@ -254,6 +259,23 @@ class MLP:
         if getattr(self, "_h", None) and self._h.value:
             lib.emt6ro_mlp_destroy(self._h)
             self._h = C.c_void_p()
+
+
+def train_many(nets, steps, batch=4096, lr=1e-3, seeds=None):
+    """`steps` Adam steps of each network in `nets` (distinct MLPs on one
+    GPU), interleaved so that the GPU overlaps them; network i draws its
+    minibatches with seeds[i].  Each ends as net.train(steps, batch, lr,
+    seeds[i]) would leave it.  Returns the (len(nets), steps) losses."""
+    nets = list(nets)
+    seeds = np.ascontiguousarray(seeds if seeds is not None else np.arange(len(nets)), np.uint64)
+    if len(seeds) != len(nets):
+        raise ValueError("one seed per network")
+    losses = np.zeros((len(nets), max(int(steps), 1)), np.float32)
+    handles = (C.c_void_p * len(nets))(*[n._h.value for n in nets])
+    check_mlp(lib.emt6ro_mlp_train_many(C.cast(handles, C.c_void_p), len(nets), int(steps),
+                                        int(batch), float(lr),
+                                        seeds.ctypes.data_as(C.c_void_p), _fp(losses)))
+    return losses[:, :int(steps)]
 
 
 class Surrogate:
@@ -285,11 +307,12 @@ class Surrogate:
         return (np.asarray(X, np.float32) - self.x_mu) / self.x_sd
 
     def _train_all(self, Xn, yn, steps, batch, lr, seed):
-        losses = []
-        for m, net in enumerate(self.nets):
+        # The members train together (train_many), each on its own minibatch
+        # stream; the result is the same as training them one by one.
+        for net in self.nets:
             net.set_data(Xn, yn)
-            losses.append(net.train(steps, batch=batch, lr=lr, seed=seed + 104729 * m))
-        return np.mean(losses, axis=0)
+        seeds = [seed + 104729 * m for m in range(len(self.nets))]
+        return train_many(self.nets, steps, batch=batch, lr=lr, seeds=seeds).mean(axis=0)
 
     def fit(self, X, y, steps=300, batch=2048, lr=1e-3, seed=0):
         """Returns the members' mean training loss per step."""

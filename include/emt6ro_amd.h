@@ -209,6 +209,22 @@ int emt6ro_mlp_set_data(emt6ro_mlp *m, const float *X, const float *y, int32_t n
  * (may be NULL) receives each step's mean squared error. */
 int emt6ro_mlp_train(emt6ro_mlp *m, int32_t steps, int32_t batch, float lr, uint64_t seed,
                      float *losses);
+/* `steps` Adam steps of each of `count` distinct networks on one device,
+ * network i drawing its minibatches with seeds[i]: step k of every network
+ * is enqueued (on the network's own stream) before step k + 1 of any, so
+ * the GPU overlaps the networks' kernels; with emt6ro_mlp_set_train_graph
+ * (nets[0], g > 0), g steps of each network are captured once into a graph
+ * of its own and the graphs are launched in turn.  Each network ends
+ * exactly as emt6ro_mlp_train with its seed would leave it.  losses (count x steps,
+ * may be NULL) receives each step's mean squared error.  No reference
+ * counterpart (the surrogate has none); used to train an ensemble. */
+int emt6ro_mlp_train_many(emt6ro_mlp *const *nets, int32_t count, int32_t steps, int32_t batch,
+                          float lr, const uint64_t *seeds, float *losses);
+/* How emt6ro_mlp_train enqueues its steps: steps_per_graph > 0 captures the
+ * kernels of that many steps once per call into a HIP graph and replays it,
+ * 0 (the default) launches every kernel directly.  Results are identical.
+ * No reference counterpart (the surrogate has none). */
+int emt6ro_mlp_set_train_graph(emt6ro_mlp *m, int32_t steps_per_graph);
 int emt6ro_mlp_predict(emt6ro_mlp *m, const float *X, int32_t n, float *y);
 /* emt6ro_mlp_predict for callers that cannot allocate device memory (code
  * under HIP graph capture, tight screening loops, harnesses that forbid

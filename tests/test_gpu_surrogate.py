@@ -146,6 +146,79 @@ def test_training_reduces_loss_and_generalises(S):
     assert mae < 0.25 * base
 
 
+def test_graph_replayed_training_equals_direct_launches(S):
+    """With set_train_graph(g), train() replays a captured graph of g steps
+    and enqueues the remainder directly; with the graph off (the default)
+    every kernel is launched directly.  Same seeds, same results, bit for
+    bit, over three calls with eager work (predict) and host synchronisation
+    between them; the graph is replayed several times per call."""
+    log("[surrogate] graph vs direct")
+    X, t = _data(6000, 21)
+    nets = [S.MLP(N_IN, HID, seed=8), S.MLP(N_IN, HID, seed=8)]
+    for n in nets:
+        n.set_data(X, t)
+    for steps, g, seed in ((37, 8, 1), (20, 8, 2), (11, 3, 3)):
+        nets[0].set_train_graph(g)
+        la = nets[0].train(steps, batch=1024, lr=1e-3, seed=seed)
+        lb = nets[1].train(steps, batch=1024, lr=1e-3, seed=seed)
+        np.testing.assert_array_equal(la, lb)
+        np.testing.assert_array_equal(nets[0].get_params(), nets[1].get_params())
+        np.testing.assert_array_equal(nets[0].predict(X[:300]), nets[1].predict(X[:300]))
+    assert np.isfinite(la).all()
+
+
+def test_adam_clock_carries_across_training_calls(S):
+    """With a one-row data set every minibatch is the same, so 20 steps in one
+    call and 10 + 10 steps in two calls must give the same weights bit for
+    bit: the second call continues the bias corrections at step 11."""
+    log("[surrogate] adam clock")
+    X, t = _data(1, 23)
+    Xr, tr = np.repeat(X, 64, axis=0), np.repeat(t, 64)
+    a, b = S.MLP(N_IN, HID, seed=9), S.MLP(N_IN, HID, seed=9)
+    for n in (a, b):
+        n.set_data(Xr, tr)
+    la = a.train(20, batch=32, lr=1e-3, seed=4)
+    lb = np.concatenate([b.train(10, batch=32, lr=1e-3, seed=4),
+                         b.train(10, batch=32, lr=1e-3, seed=7)])
+    np.testing.assert_array_equal(la, lb)
+    np.testing.assert_array_equal(a.get_params(), b.get_params())
+    # After set_params the clock starts again at step 1: a network that has
+    # taken 20 steps matches a fresh one given the same parameters.
+    p = a.get_params()
+    c = S.MLP(N_IN, HID, seed=3)
+    c.set_data(Xr, tr)
+    for n in (a, c):
+        n.set_params(p)
+    np.testing.assert_array_equal(a.train(3, batch=32, lr=1e-3, seed=4),
+                                  c.train(3, batch=32, lr=1e-3, seed=4))
+    np.testing.assert_array_equal(a.get_params(), c.get_params())
+
+
+def test_train_many_equals_training_one_by_one(S):
+    """train_many interleaves the steps of several networks on their own
+    streams (directly, then as replayed graphs of 4 steps); each network
+    must end exactly as training it alone with its seed leaves it, over
+    three calls (the Adam clocks carry on)."""
+    log("[surrogate] train_many")
+    X, t = _data(5000, 31)
+    a = [S.MLP(N_IN, HID, seed=20 + i) for i in range(3)]
+    b = [S.MLP(N_IN, HID, seed=20 + i) for i in range(3)]
+    for n in a + b:
+        n.set_data(X, t)
+    for steps, base, g in ((25, 100, 0), (9, 200, 0), (10, 300, 4)):
+        b[0].set_train_graph(g)
+        la = np.stack([n.train(steps, batch=512, lr=1e-3, seed=base + i) for i, n in enumerate(a)])
+        lb = S.train_many(b, steps, batch=512, lr=1e-3, seeds=[base, base + 1, base + 2])
+        np.testing.assert_array_equal(la, lb)
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x.get_params(), y.get_params())
+    assert not np.array_equal(a[0].get_params(), a[1].get_params())
+    with pytest.raises(Exception):
+        S.train_many([b[0], b[0]], 2, batch=64, seeds=[1, 2])
+    with pytest.raises(ValueError):
+        S.train_many(b, 2, batch=64, seeds=[1])
+
+
 def test_surrogate_standardises_and_errors(S):
     log("[surrogate] wrapper")
     rng = np.random.default_rng(0)
