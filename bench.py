@@ -77,6 +77,10 @@ def parse():
     ap.add_argument("--surrogate-graph", type=int, default=None,
                     help="training steps per replayed HIP graph (0: direct launches; "
                          "default: the library's)")
+    ap.add_argument("--surrogate-bounds", action="store_true",
+                    help="only measure what bounds a training step at batch 4096: a grid-wide "
+                         "barrier inside a kernel against a kernel launch, and the library "
+                         "GEMM (torch.matmul, hipBLASLt) on the step's product shapes")
     ap.add_argument("--json-out", default=None)
     return ap.parse_args()
 
@@ -196,6 +200,51 @@ def surrogate_sample(train_steps=200, batch=4096, n_data=200000, n_screen=100000
             "note": "host-timed; includes the host<->device copies of predict"}
 
 
+def surrogate_bounds(batch=4096, hidden=512, reps=500):
+    """What bounds a surrogate training step at batch 4096.  (1) A barrier
+    across the whole GPU inside one kernel (one workgroup per CU) against a
+    kernel boundary, both with the same writes between them: whether the
+    step's 12 launches could become one kernel.  (2) The vendor library's
+    GEMM (torch.matmul on bf16, which calls hipBLASLt) on the step's product
+    shapes, timed per product with events; rocprofv3 --kernel-trace gives the
+    kernels' own durations."""
+    import torch
+    from emt6ro._native import probe_grid_barrier
+    from emt6ro.surrogate import N_FEATURES
+    out = {"grid_barrier": []}
+    for mode, name in ((0, "one counter"), (1, "8 group counters")):
+        for words in (0, 1024, 16384):
+            r = probe_grid_barrier(mode=mode, words=words)
+            out["grid_barrier"].append({"mode": name, "bytes_per_workgroup": 4 * words,
+                                        "barrier_us": r["barrier_us"],
+                                        "launch_us": r["launch_us"]})
+    kin = (N_FEATURES + 31) // 32 * 32
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(0)
+    shapes = [("forward layer 1", batch, kin, hidden), ("forward layers 2-3, input gradients",
+                                                         batch, hidden, hidden),
+              ("weight gradients of layers 2-3", hidden, batch, hidden),
+              ("weight gradient of layer 1", hidden, batch, kin)]
+    out["library_gemm"] = []
+    for name, M, K, N in shapes:
+        a = torch.randn(M, K, device=dev, dtype=torch.bfloat16, generator=g)
+        bt = torch.randn(N, K, device=dev, dtype=torch.bfloat16, generator=g)
+        c = torch.empty(M, N, device=dev, dtype=torch.bfloat16)
+        for _ in range(20):
+            torch.matmul(a, bt.t(), out=c)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            torch.matmul(a, bt.t(), out=c)
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        out["library_gemm"].append({"product": name, "M": M, "K": K, "N": N,
+                                    "us_per_call": us, "tflops": 2 * M * N * K / us / 1e6})
+    return out
+
+
 def copy_calibration(gib=1):
     """A plain device copy of known size, to scale rocprofv3's FETCH_SIZE and
     WRITE_SIZE counters (run under the profiler with --copy-calibrate)."""
@@ -232,6 +281,9 @@ def main():
     args = parse()
     if args.copy_calibrate:
         copy_calibration()
+        return
+    if args.surrogate_bounds:
+        print(json.dumps({"surrogate_bounds": surrogate_bounds()}), flush=True)
         return
     if args.surrogate_only:
         print(json.dumps({"surrogate": surrogate_sample(graph_steps=args.surrogate_graph)}),

@@ -5,6 +5,7 @@
 // each workgroup records the shader-clock and the constant-rate wall-clock
 // ticks its loop took, so the probe reports both bytes per second and bytes
 // per shader clock per CU.  No reference counterpart (measurement only).
+// Also a grid-barrier probe (below).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -65,8 +66,168 @@ __global__ __launch_bounds__(kProbeThreads) void lds_probe_kernel(uint32_t *sink
   }
 }
 
+// ---------------------------------------------------------------------------
+// Grid-barrier probe: what one workgroup-wide barrier across the whole GPU
+// costs inside a kernel, against ending one kernel and starting the next.
+// One workgroup per CU (a cooperative launch, so all are resident) runs
+// 2 * kPhases barriers; before the first barrier of a phase each workgroup
+// writes `words` words of its own slice (dirty lines for the release fence
+// to write back), after it reads the slice of workgroup (b + 37) % G and
+// checks every word (visibility across XCDs).  Every wait is bounded by
+// the wall clock (1 s); a workgroup that times out raises an error flag
+// that makes all the others leave too.
+// ---------------------------------------------------------------------------
+constexpr int kLine = 32;        // words per 128-byte line
+constexpr int kBarPhases = 100;
+struct GridBar {
+  unsigned central[kLine];
+  unsigned top[kLine];
+  unsigned grp[8 * kLine];       // per-group arrival counters, one line each
+  unsigned rel[8 * kLine];       // per-group release words
+  unsigned err[kLine];
+  unsigned bad[kLine];
+};
+
+__device__ __forceinline__ unsigned bar_load(unsigned *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void bar_store(unsigned *p, unsigned v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned bar_add(unsigned *p) {
+  return __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits until *p >= v; false on the time limit or another workgroup's error.
+__device__ __forceinline__ bool bar_wait(unsigned *p, unsigned v, GridBar *bar) {
+  const uint64_t t0 = wall_clock64();
+  while (bar_load(p) < v) {
+    if (bar_load(bar->err) != 0u || wall_clock64() - t0 > 100000000ull) {
+      bar_store(bar->err, 1u);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
+// Barrier number k (1, 2, ...).  mode 0: one counter that every workgroup
+// increments and polls; mode 1: eight group counters, the last arrival of a
+// group increments a top counter, the last of those writes the eight group
+// release words, and each workgroup polls its group's.  Both publish the
+// workgroup's writes with an agent-scope release fence before arriving and
+// take the others' with an acquire fence after leaving.
+__device__ __forceinline__ bool grid_barrier(GridBar *bar, unsigned k, int mode) {
+  const int G = (int)gridDim.x, g = (int)blockIdx.x & 7;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __syncthreads();
+  __shared__ int ok;
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (mode == 0) {
+      bar_add(bar->central);
+      ok = bar_wait(bar->central, k * (unsigned)G, bar);
+    } else {
+      const unsigned gsize = (unsigned)((G - g + 7) / 8);
+      if (bar_add(&bar->grp[g * kLine]) == k * gsize - 1u) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+        if (bar_add(bar->top) == k * 8u - 1u) {
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+          for (int j = 0; j < 8; ++j) bar_store(&bar->rel[j * kLine], k);
+        }
+      }
+      ok = bar_wait(&bar->rel[g * kLine], k, bar);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+__global__ __launch_bounds__(256) void grid_barrier_kernel(GridBar *bar, uint32_t *buf, int words,
+                                                           int mode) {
+  const int G = (int)gridDim.x, b = (int)blockIdx.x;
+  unsigned nbad = 0, k = 0;
+  for (int p = 0; p < kBarPhases; ++p) {
+    uint32_t *mine = buf + (size_t)b * words;
+    for (int i = threadIdx.x; i < words; i += blockDim.x) mine[i] = (uint32_t)(p * G + b);
+    if (!grid_barrier(bar, ++k, mode)) break;
+    const int o = (b + 37) % G;
+    const uint32_t *other = buf + (size_t)o * words;
+    for (int i = threadIdx.x; i < words; i += blockDim.x)
+      if (other[i] != (uint32_t)(p * G + o)) ++nbad;
+    if (!grid_barrier(bar, ++k, mode)) break;
+  }
+  if (nbad) atomicAdd(bar->bad, nbad);
+}
+
+// The same writes, one phase per launch.
+__global__ __launch_bounds__(256) void phase_write_kernel(uint32_t *buf, int words, int p) {
+  uint32_t *mine = buf + (size_t)blockIdx.x * words;
+  for (int i = threadIdx.x; i < words; i += blockDim.x) mine[i] = (uint32_t)(p * (int)gridDim.x + (int)blockIdx.x);
+}
+
 }  // namespace
 }  // namespace emt6ro
+
+extern "C" int emt6ro_probe_grid_barrier(int32_t device, int32_t mode, int32_t words,
+                                         double out[2]) {
+  using namespace emt6ro;
+  constexpr int kMaxWords = 16384;  // 64 KB per workgroup
+  if (!out || (mode != 0 && mode != 1) || words < 0 || words > kMaxWords) return EMT6RO_ERR_ARG;
+  hipError_t e;
+  if ((e = hipSetDevice(device)) != hipSuccess) return EMT6RO_ERR_DEVICE;
+  int cus = 0, coop = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device) != hipSuccess)
+    return EMT6RO_ERR_DEVICE;
+  if (!coop || cus < 1) return EMT6RO_ERR_DEVICE;
+  const int G = cus;
+  GridBar *bar = nullptr;
+  uint32_t *buf = nullptr;
+  if (hipMalloc(&bar, sizeof(GridBar)) != hipSuccess) return EMT6RO_ERR_DEVICE;
+  if (hipMalloc(&buf, (size_t)G * kMaxWords * 4) != hipSuccess) {
+    (void)hipFree(bar);
+    return EMT6RO_ERR_DEVICE;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  float ms_bar = 0.f, ms_launch = 0.f;
+  GridBar hb{};
+  e = hipSuccess;
+  for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {  // the first is a warm-up
+    if ((e = hipMemset(bar, 0, sizeof(GridBar))) != hipSuccess) break;
+    if ((e = hipDeviceSynchronize()) != hipSuccess) break;
+    int w = words, m = mode;
+    void *args[] = {&bar, &buf, &w, &m};
+    (void)hipEventRecord(e0, nullptr);
+    if ((e = hipLaunchCooperativeKernel((const void *)grid_barrier_kernel, dim3(G), dim3(256), args,
+                                        0, nullptr)) != hipSuccess)
+      break;
+    (void)hipEventRecord(e1, nullptr);
+    if ((e = hipEventSynchronize(e1)) != hipSuccess) break;
+    e = hipEventElapsedTime(&ms_bar, e0, e1);
+  }
+  if (e == hipSuccess) e = hipMemcpy(&hb, bar, sizeof(GridBar), hipMemcpyDeviceToHost);
+  for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+    (void)hipEventRecord(e0, nullptr);
+    for (int p = 0; p < 2 * kBarPhases; ++p)
+      hipLaunchKernelGGL(phase_write_kernel, dim3(G), dim3(256), 0, nullptr, buf, words, p);
+    (void)hipEventRecord(e1, nullptr);
+    if ((e = hipEventSynchronize(e1)) != hipSuccess) break;
+    e = hipEventElapsedTime(&ms_launch, e0, e1);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(bar);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return EMT6RO_ERR_DEVICE;
+  if (hb.err[0] != 0u || hb.bad[0] != 0u) return EMT6RO_ERR_DEVICE;  // timed out or saw stale data
+  out[0] = ms_bar * 1e3 / (2 * kBarPhases);     // microseconds per grid barrier
+  out[1] = ms_launch * 1e3 / (2 * kBarPhases);  // microseconds per launch
+  return EMT6RO_OK;
+}
 
 extern "C" int emt6ro_probe_lds(int32_t device, int32_t width_bytes, double out[4]) {
   using namespace emt6ro;

@@ -90,6 +90,7 @@ def _load():
         "emt6ro_mlp_grads": ([P, f32p, f32p, i32, f32p, f32p], C.c_int),
         "emt6ro_mlp_destroy": ([P], None),
         "emt6ro_probe_lds": ([i32, i32, C.POINTER(C.c_double)], C.c_int),
+        "emt6ro_probe_grid_barrier": ([i32, i32, i32, C.POINTER(C.c_double)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -118,6 +119,17 @@ def probe_lds(width_bytes=16, device=0):
         raise RuntimeError(f"emt6ro_probe_lds failed ({rc})")
     return {"bytes_per_s": out[0], "bytes_per_clk_cu": out[1], "clock_hz": out[2],
             "cus": int(out[3])}
+
+
+def probe_grid_barrier(mode=1, words=0, device=0):
+    """Microseconds per grid-wide barrier inside a kernel (one workgroup per
+    CU) and per back-to-back kernel launch doing the same work
+    (emt6ro_probe_grid_barrier)."""
+    out = (C.c_double * 2)()
+    rc = lib.emt6ro_probe_grid_barrier(int(device), int(mode), int(words), out)
+    if rc != 0:
+        raise RuntimeError(f"emt6ro_probe_grid_barrier failed ({rc})")
+    return {"barrier_us": out[0], "launch_us": out[1]}
 
 
 def check_mlp(rc):

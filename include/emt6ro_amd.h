@@ -269,6 +269,15 @@ int emt6ro_instr_enabled(void);
  * or 16).  out[0] bytes/s over the GPU, out[1] bytes per shader clock per CU,
  * out[2] the shader clock (Hz) under the probe, out[3] the CU count. */
 int emt6ro_probe_lds(int32_t device, int32_t width_bytes, double out[4]);
+/* What a barrier across every workgroup of a kernel costs against a kernel
+ * boundary: one workgroup per CU (cooperative launch) runs 200 barriers,
+ * each workgroup writing `words` (0..16384) words of its own between them
+ * and checking another's after; mode 0 one arrival counter, mode 1 eight
+ * group counters and group release words, both with agent-scope fences.
+ * out[0] microseconds per barrier, out[1] microseconds per launch of a
+ * kernel doing the same writes back to back.  Fails if a wait timed out (1
+ * s) or stale data was read. */
+int emt6ro_probe_grid_barrier(int32_t device, int32_t mode, int32_t words, double out[2]);
 
 #ifdef __cplusplus
 }

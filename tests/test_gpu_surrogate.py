@@ -219,6 +219,28 @@ def test_train_many_equals_training_one_by_one(S):
         S.train_many(b, 2, batch=64, seeds=[1])
 
 
+def test_train_many_at_full_size(S):
+    """The GA's ensemble shape: 5 networks of N_FEATURES-512-512-512-1 at
+    batch 4096 (64 x 64 tiles, split-K weight gradients) trained together
+    end exactly as trained one by one; so do 8 networks and a list mixing
+    two shapes."""
+    log("[surrogate] train_many full size")
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((20000, max(S.N_FEATURES, N_IN + 40))).astype(np.float32)
+    t = (X[:, 0] + 0.5 * X[:, 1] * X[:, 2]).astype(np.float32)
+    for shapes, batch in (([(S.N_FEATURES, 512)] * 5, 4096), ([(N_IN, HID)] * 8, 256),
+                          ([(N_IN, HID), (N_IN + 40, 256), (N_IN, HID)], 256)):
+        a = [S.MLP(k, hid, seed=40 + i) for i, (k, hid) in enumerate(shapes)]
+        b = [S.MLP(k, hid, seed=40 + i) for i, (k, hid) in enumerate(shapes)]
+        for n, (k, _) in zip(a + b, shapes + shapes):
+            n.set_data(np.ascontiguousarray(X[:, :k]), t)
+        la = np.stack([n.train(3, batch=batch, lr=1e-3, seed=7 + i) for i, n in enumerate(a)])
+        lb = S.train_many(b, 3, batch=batch, lr=1e-3, seeds=[7 + i for i in range(len(b))])
+        np.testing.assert_array_equal(la, lb)
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x.get_params(), y.get_params())
+
+
 def test_surrogate_standardises_and_errors(S):
     log("[surrogate] wrapper")
     rng = np.random.default_rng(0)
