@@ -3,6 +3,7 @@ comparison, as a command line tool.
 
   python -m emt6ro.ga.experiment data --out data.npz [--protocols 1200]
   python -m emt6ro.ga.experiment compare --data data.npz [more.npz ...] --out result.json
+  python -m emt6ro.ga.experiment config4 --data data.npz [more.npz ...] --out result.json
 
 `data` simulates random dose protocols on synthetic 53x53 tumours (disk radii
 8-14) and records, per (protocol, tumour), the surrogate features and the mean
@@ -14,6 +15,10 @@ half from the GA's own initial-population generator (python_ga/utils.py:
 `compare` trains the surrogate on 90% of the samples, reports its MAE on the
 other 10%, then runs the simulator-only GA and the surrogate-screened GA on
 three of the tumours and reports best fitness against wall time.
+
+`config4` trains the configured 152-input network (and, beside it, the GA's
+feature network) at batch 4096 on 90% of the protocols and reports the
+held-out error against the simulator.
 """
 import argparse
 import json
@@ -138,6 +143,54 @@ def train_surrogate(X, y, pi, a, params=None):
     return model, out
 
 
+def cmd_config4(a):
+    """The surrogate-training workload as configured: the 152-512-512-512-1
+    network on the raw input (144 dose slots in Gy + the 8 tumour
+    descriptors), batch 4096, trained on 90% of the protocols of the `data`
+    files and scored by its mean absolute error against the simulated mean
+    on the other 10%; beside it the GA's feature network and the
+    predict-the-mean baseline.  Each network is fitted from scratch for
+    every step count in --steps."""
+    params = sim.load_parameters(a.params)
+    X, y, pi, z = load_samples(a.data, params)
+    mean = np.concatenate([np.load(p)["mean"] for p in a.data])
+    genomes = np.concatenate([np.load(p)["genomes"] for p in a.data]).astype(np.float32)
+    desc = np.asarray(z["descriptors"], np.float32)
+    P, T = mean.shape
+    ti = np.tile(np.arange(T), P)
+    Xr = np.concatenate([desc[ti], genomes[pi]], axis=1)
+    rng = np.random.default_rng(a.seed)
+    te = (rng.random(P) < 0.1)[pi]
+    base = float(np.abs(y[te] - y[~te].mean()).mean())
+    ym = float(y[te].mean())
+    out = {"samples": int(len(y)), "train_samples": int((~te).sum()),
+           "test_samples": int(te.sum()), "protocols": int(P), "tumours": int(T),
+           "replicates": int(z["reps"]), "hours": int(z["hours"]), "batch": a.batch,
+           "lr": a.lr, "test_mean": ym, "predict_the_mean_mae": base, "runs": []}
+    log(f"[config4] {len(y)} samples ({int((~te).sum())} train, {int(te.sum())} held out), "
+        f"test mean {ym:.1f}, predict-the-mean MAE {base:.2f}")
+    for name, Xi, members in (("raw 152 inputs", Xr, 1), ("raw 152 inputs", Xr, 5),
+                              ("71 features", X, 5)):
+        for steps in a.steps:
+            model = S.Surrogate(in_features=Xi.shape[1], seed=a.seed + 1, members=members,
+                                params=params)
+            t0 = time.perf_counter()
+            losses = model.fit(Xi[~te], y[~te], steps=steps, batch=a.batch, lr=a.lr,
+                               seed=a.seed + 1)
+            sec = time.perf_counter() - t0
+            mae = float(np.abs(model.predict(Xi[te]) - y[te]).mean())
+            r = {"input": name, "in_features": int(Xi.shape[1]), "members": members,
+                 "steps": steps, "fit_seconds": sec, "final_loss": float(losses[-20:].mean()),
+                 "mae": mae, "mae_fraction_of_mean": mae / ym}
+            out["runs"].append(r)
+            log(f"[config4] {name}, {members} network(s), {steps} steps: held-out MAE "
+                f"{mae:.2f} = {100 * mae / ym:.2f}% ({sec:.1f} s)")
+            del model
+    with open(a.out, "w") as f:
+        json.dump(out, f, indent=1)
+    log(f"[config4] -> {a.out}")
+
+
 def cmd_compare(a):
     params = sim.load_parameters(a.params)
     X, y, pi, z = load_samples(a.data, params)
@@ -236,8 +289,16 @@ def main(argv=None):
                    help="replicates per tumour when re-simulating each run's best genome")
     c.add_argument("--seed", type=int, default=0)
     c.add_argument("--params", default=DEFAULT_PARAMS)
+    f = sub.add_parser("config4")
+    f.add_argument("--data", required=True, nargs="+", help="data files of the same tumours")
+    f.add_argument("--out", required=True)
+    f.add_argument("--steps", type=int, nargs="+", default=[1000, 4000])
+    f.add_argument("--batch", type=int, default=4096)
+    f.add_argument("--lr", type=float, default=1e-3)
+    f.add_argument("--seed", type=int, default=0)
+    f.add_argument("--params", default=DEFAULT_PARAMS)
     a = p.parse_args(argv)
-    {"data": cmd_data, "compare": cmd_compare}[a.cmd](a)
+    {"data": cmd_data, "compare": cmd_compare, "config4": cmd_config4}[a.cmd](a)
 
 
 if __name__ == "__main__":
