@@ -166,26 +166,28 @@ def cmd_config4(a):
     out = {"samples": int(len(y)), "train_samples": int((~te).sum()),
            "test_samples": int(te.sum()), "protocols": int(P), "tumours": int(T),
            "replicates": int(z["reps"]), "hours": int(z["hours"]), "batch": a.batch,
-           "lr": a.lr, "test_mean": ym, "predict_the_mean_mae": base, "runs": []}
+           "test_mean": ym, "predict_the_mean_mae": base, "runs": []}
     log(f"[config4] {len(y)} samples ({int((~te).sum())} train, {int(te.sum())} held out), "
         f"test mean {ym:.1f}, predict-the-mean MAE {base:.2f}")
-    for name, Xi, members in (("raw 152 inputs", Xr, 1), ("raw 152 inputs", Xr, 5),
-                              ("71 features", X, 5)):
-        for steps in a.steps:
-            model = S.Surrogate(in_features=Xi.shape[1], seed=a.seed + 1, members=members,
-                                params=params)
-            t0 = time.perf_counter()
-            losses = model.fit(Xi[~te], y[~te], steps=steps, batch=a.batch, lr=a.lr,
-                               seed=a.seed + 1)
-            sec = time.perf_counter() - t0
-            mae = float(np.abs(model.predict(Xi[te]) - y[te]).mean())
-            r = {"input": name, "in_features": int(Xi.shape[1]), "members": members,
-                 "steps": steps, "fit_seconds": sec, "final_loss": float(losses[-20:].mean()),
-                 "mae": mae, "mae_fraction_of_mean": mae / ym}
-            out["runs"].append(r)
-            log(f"[config4] {name}, {members} network(s), {steps} steps: held-out MAE "
-                f"{mae:.2f} = {100 * mae / ym:.2f}% ({sec:.1f} s)")
-            del model
+    nets = [("raw 152 inputs", Xr, 1), ("raw 152 inputs", Xr, 5), ("71 features", X, 5)]
+    for name, Xi, members in nets[:a.nets]:
+        for lr in a.lr:
+            for steps in a.steps:
+                model = S.Surrogate(in_features=Xi.shape[1], seed=a.seed + 1, members=members,
+                                    params=params)
+                t0 = time.perf_counter()
+                losses = model.fit(Xi[~te], y[~te], steps=steps, batch=a.batch, lr=lr,
+                                   seed=a.seed + 1)
+                sec = time.perf_counter() - t0
+                mae = float(np.abs(model.predict(Xi[te]) - y[te]).mean())
+                r = {"input": name, "in_features": int(Xi.shape[1]), "members": members,
+                     "steps": steps, "lr": lr, "fit_seconds": sec,
+                     "final_loss": float(losses[-20:].mean()),
+                     "mae": mae, "mae_fraction_of_mean": mae / ym}
+                out["runs"].append(r)
+                log(f"[config4] {name}, {members} network(s), lr {lr:g}, {steps} steps: "
+                    f"held-out MAE {mae:.2f} = {100 * mae / ym:.2f}% ({sec:.1f} s)")
+                del model
     with open(a.out, "w") as f:
         json.dump(out, f, indent=1)
     log(f"[config4] -> {a.out}")
@@ -294,7 +296,10 @@ def main(argv=None):
     f.add_argument("--out", required=True)
     f.add_argument("--steps", type=int, nargs="+", default=[1000, 4000])
     f.add_argument("--batch", type=int, default=4096)
-    f.add_argument("--lr", type=float, default=1e-3)
+    f.add_argument("--lr", type=float, nargs="+", default=[1e-3])
+    f.add_argument("--nets", type=int, default=3,
+                   help="networks to fit: the single 152-input one, then its ensemble of 5, "
+                        "then the feature ensemble")
     f.add_argument("--seed", type=int, default=0)
     f.add_argument("--params", default=DEFAULT_PARAMS)
     a = p.parse_args(argv)
