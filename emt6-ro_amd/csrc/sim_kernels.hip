@@ -351,17 +351,22 @@ struct DiffSel<4, NT, A, B, RS, PS> {
 // Lane-run band heights and policy by code B: 0 3/4/6, 1 4/6/8 (fewest
 // instructions); 2 4/6/8, 3 2/3/4, 4 3/4/6 (lowest bands that fit); 5 3/4/5
 // (fewest instructions by the stencil's measured count).
+// Band heights, height policy and diffusion-substep unroll (1, or 2: the
+// substeps run in pairs) of a lane-run variant.
 template <int B> struct LaneRunShape;
-template <> struct LaneRunShape<0> { static constexpr int h1 = 3, h2 = 4, h3 = 6, pol = 0; };
-template <> struct LaneRunShape<1> { static constexpr int h1 = 4, h2 = 6, h3 = 8, pol = 0; };
-template <> struct LaneRunShape<2> { static constexpr int h1 = 4, h2 = 6, h3 = 8, pol = 1; };
-template <> struct LaneRunShape<3> { static constexpr int h1 = 2, h2 = 3, h3 = 4, pol = 1; };
-template <> struct LaneRunShape<4> { static constexpr int h1 = 3, h2 = 4, h3 = 6, pol = 1; };
-template <> struct LaneRunShape<5> { static constexpr int h1 = 3, h2 = 4, h3 = 5, pol = 2; };
+template <> struct LaneRunShape<0> { static constexpr int h1 = 3, h2 = 4, h3 = 6, pol = 0, unroll = 1; };
+template <> struct LaneRunShape<1> { static constexpr int h1 = 4, h2 = 6, h3 = 8, pol = 0, unroll = 1; };
+template <> struct LaneRunShape<2> { static constexpr int h1 = 4, h2 = 6, h3 = 8, pol = 1, unroll = 1; };
+template <> struct LaneRunShape<3> { static constexpr int h1 = 2, h2 = 3, h3 = 4, pol = 1, unroll = 1; };
+template <> struct LaneRunShape<4> { static constexpr int h1 = 3, h2 = 4, h3 = 6, pol = 1, unroll = 1; };
+template <> struct LaneRunShape<5> { static constexpr int h1 = 3, h2 = 4, h3 = 5, pol = 2, unroll = 1; };
+template <> struct LaneRunShape<6> { static constexpr int h1 = 4, h2 = 6, h3 = 8, pol = 1, unroll = 2; };
+template <> struct LaneRunShape<7> { static constexpr int h1 = 3, h2 = 4, h3 = 5, pol = 2, unroll = 2; };
+template <> struct LaneRunShape<8> { static constexpr int h1 = 3, h2 = 4, h3 = 6, pol = 1, unroll = 2; };
 template <int NT, int A, int B, int RS, int PS>
 struct DiffSel<3, NT, A, B, RS, PS> {
   using S = LaneRunShape<B>;
-  using T = LaneRunDiffuser<NT, RS, PS, A, S::h1, S::h2, S::h3, S::pol>;
+  using T = LaneRunDiffuser<NT, RS, PS, A, S::h1, S::h2, S::h3, S::pol, S::unroll>;
 };
 
 
@@ -642,7 +647,8 @@ hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds
 // columns per lane),
 // workgroup size, A, B (kind 0: band rows, items per thread; kind 1: Jacobi
 // and final sites per thread; kind 3: A = lane slots per thread, B = band
-// heights and policy (LaneRunShape); kind 4:
+// heights, policy and substep unroll (LaneRunShape; 45-48 are 32, 44, 43
+// and 39 with the substeps in pairs); kind 4:
 // A = lane slots per thread, B = heights 4/6/8 (0) or 2/4/8 preferring low
 // bands (1)), ring-save
 // slots per thread, LDS row stride (lattices up to stride-2 on a side),
@@ -693,7 +699,11 @@ hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds
   X(41, 3, 256, 2, 4, 2, 36, 4)   \
   X(42, 3, 128, 4, 4, 3, 36, 4)   \
   X(43, 3, 128, 2, 2, 3, 36, 4)   \
-  X(44, 3, 256, 3, 5, 2, 56, 4)
+  X(44, 3, 256, 3, 5, 2, 56, 4)   \
+  X(45, 3, 256, 2, 6, 2, 56, 4)   \
+  X(46, 3, 256, 3, 7, 2, 56, 4)   \
+  X(47, 3, 128, 2, 6, 3, 36, 4)   \
+  X(48, 3, 512, 3, 8, 2, 88, 2)
 
 const VariantInfo *variant_table(int *count) {
   static const VariantInfo table[] = {

@@ -579,7 +579,10 @@ struct TempGrid {
 // lanes and reads only its own column from LDS.
 // `kept`: a TempGrid the caller prepared for this ROI and keeps between
 // steps; without it one is set up for this step.
-template <int NT, int L, int KMAX, int RS, int PS, bool LANES = false>
+// UNROLL = 2: the substeps before the final one run in pairs, two copies of
+// the substep body per loop pass (the compiler does not unroll a loop with
+// barriers by itself); results are the same.
+template <int NT, int L, int KMAX, int RS, int PS, bool LANES = false, int UNROLL = 1>
 __device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5],
                         const uint32_t jac[KMAX], const uint32_t fin[KMAX], uint32_t &updates,
                         TempGrid<NT, RS, PS> *kept = nullptr) {
@@ -604,8 +607,7 @@ __device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5]
   // Mask-box (0, 0) is padded (r0, c0).
   const int origin = r0 * RS + c0;
   __syncthreads();
-  for (int step = 0; step < total; ++step) {
-    const bool final_sub = (step == total - 1);
+  auto substep = [&](const bool final_sub) {
     float res[KMAX][L][3];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
@@ -663,7 +665,14 @@ __device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5]
       }
     }
     __syncthreads();
-  }
+  };
+  int step = 0;
+  if (UNROLL == 2)
+    for (; step + 2 < total; step += 2) {
+      substep(false);
+      substep(false);
+    }
+  for (; step < total; ++step) substep(step == total - 1);
   tg.restore(sub, PV);
   __syncthreads();
 }
@@ -856,7 +865,7 @@ struct LaneRuns {
 // Band heights H1 < H2 < H3 are tried per ROI; layouts of height H3 always
 // fit the KCAP lane slots.
 template <int NT, int RS, int PS, int KCAP, int H1 = 3, int H2 = 4, int H3 = 6,
-          int POLICY = 0>
+          int POLICY = 0, int UNROLL = 1>
 struct LaneRunDiffuser {
   using LR = LaneRuns<NT, RS, KCAP>;
   static_assert(KCAP * NT >= LR::kMaxLanes(H3), "the tallest bands must fit");
@@ -961,7 +970,7 @@ struct LaneRunDiffuser {
     uint32_t j2[K], f2[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) { j2[k] = jac[k]; f2[k] = fin[k]; }
-    diffuse<NT, LL, K, RS, PS, true>(p, sub, PV, roi, j2, f2, upd, &tg);
+    diffuse<NT, LL, K, RS, PS, true, UNROLL>(p, sub, PV, roi, j2, f2, upd, &tg);
   }
 
   __device__ void run(const DevParams &p, float *sub, int PV, const int roi[5], uint32_t &upd) {
