@@ -620,30 +620,29 @@ __device__ void diffuse(const DevParams &p, float *sub, int PV, const int roi[5]
       for (int s = 0; s < 3; ++s) {
         const float *P = sub + s * PV + a0;
         const float kk = p.k[s];
-        // Rolling window down the column: centre values of the rows above,
-        // at and below, and their horizontal neighbour sums hu, hm, hd.
-        float uc, hu, mc, hm;
-        if (LANES) {
-          uc = P[0];
+        // Rolling window down the column: G(r) = fma(H(r), 1/sqrt2, C(r))
+        // of the rows above (gu) and at (gm) the output row, its centre mc
+        // and horizontal neighbour sum hm; each row's H and G once.
+        float gu, mc, hm, gm;
+        {
+          const float uc = P[0];
           mc = P[RS];
-          hu = from_left_lane(uc) + from_right_lane(uc);
-          hm = from_left_lane(mc) + from_right_lane(mc);
-        } else {
-          uc = P[0]; hu = P[-1] + P[1];
-          mc = P[RS]; hm = P[RS - 1] + P[RS + 1];
+          const float hu = LANES ? from_left_lane(uc) + from_right_lane(uc) : P[-1] + P[1];
+          hm = LANES ? from_left_lane(mc) + from_right_lane(mc) : P[RS - 1] + P[RS + 1];
+          gu = fmaf(hu, R2, uc);
+          gm = fmaf(hm, R2, mc);
         }
 #pragma unroll
         for (int l = 0; l < L; ++l) {
           const float *dn = P + (l + 2) * RS;
           const float dc = dn[0];
           const float hd = LANES ? from_left_lane(dc) + from_right_lane(dc) : dn[-1] + dn[1];
-          const float o = (uc + dc) + hm;
-          const float dd = hu + hd;
-          float Lp = fmaf(dd, R2, o);
+          const float gd = fmaf(hd, R2, dc);
+          float Lp = (gu + gd) + hm;
           Lp = fmaf(-F, mc, Lp);
           res[k][l][s] = fmaf(kk, Lp, mc);
-          uc = mc; hu = hm;
-          mc = dc; hm = hd;
+          gu = gm;
+          mc = dc; hm = hd; gm = gd;
         }
       }
     }
@@ -1088,22 +1087,23 @@ __device__ void diffuse_pairs(const DevParams &p, float *sub, int PV, const int 
         const float kk = p.k[s];
         const float2 u = P[0], m = P[RS / 2];
         // Centre values and horizontal sums of the rows above, at and below.
-        float uca = u.x, ucb = u.y, mca = m.x, mcb = m.y;
-        float hua = from_left_lane(u.y) + u.y, hub = u.x + from_right_lane(u.x);
+        float mca = m.x, mcb = m.y;
         float hma = from_left_lane(m.y) + m.y, hmb = m.x + from_right_lane(m.x);
+        float gua = fmaf(from_left_lane(u.y) + u.y, R2, u.x);
+        float gub = fmaf(u.x + from_right_lane(u.x), R2, u.y);
+        float gma = fmaf(hma, R2, mca), gmb = fmaf(hmb, R2, mcb);
 #pragma unroll
         for (int l = 0; l < L; ++l) {
           const float2 d = P[(l + 2) * (RS / 2)];
           const float hda = from_left_lane(d.y) + d.y, hdb = d.x + from_right_lane(d.x);
-          const float oa = (uca + d.x) + hma, ob = (ucb + d.y) + hmb;
-          const float dda = hua + hda, ddb = hub + hdb;
-          float La = fmaf(dda, R2, oa), Lb = fmaf(ddb, R2, ob);
+          const float gda = fmaf(hda, R2, d.x), gdb = fmaf(hdb, R2, d.y);
+          float La = (gua + gda) + hma, Lb = (gub + gdb) + hmb;
           La = fmaf(-F, mca, La);
           Lb = fmaf(-F, mcb, Lb);
           ra[k][l][s] = fmaf(kk, La, mca);
           rb[k][l][s] = fmaf(kk, Lb, mcb);
-          uca = mca; ucb = mcb; hua = hma; hub = hmb;
-          mca = d.x; mcb = d.y; hma = hda; hmb = hdb;
+          gua = gma; gub = gmb;
+          mca = d.x; mcb = d.y; hma = hda; hmb = hdb; gma = gda; gmb = gdb;
         }
       }
     }
@@ -1331,9 +1331,8 @@ __device__ __forceinline__ void pp_substep(const DevParams &p, const float *src,
       const float hu = P[-RS - 1] + P[-RS + 1];
       const float hm = P[-1] + P[1];
       const float hd = P[RS - 1] + P[RS + 1];
-      const float o = (P[-RS] + P[RS]) + hm;
-      const float dd = hu + hd;
-      float Lp = fmaf(dd, R2, o);
+      const float gu = fmaf(hu, R2, P[-RS]), gd = fmaf(hd, R2, P[RS]);
+      float Lp = (gu + gd) + hm;
       Lp = fmaf(-F, mc, Lp);
       dst[s * PV + site[k]] = fmaf(p.k[s], Lp, mc);
     }

@@ -421,11 +421,12 @@ void eo_diffuse_f64(const eo_params *p, int H, int W, const double *sub_in,
 }
 
 /* FP32 stencil in the build's fixed order, with H(r) = left(r) + right(r)
- * the sum of a row's two horizontal neighbours of the centre column:
- *   o = (up + down) + H(mid);  d = H(up row) + H(down row)
- *   L = fma(d, 1/sqrt2, o);  L = fma(-F, C, L);  C' = fma(k, L, C)
- * (a column walking down the rows computes each H once and reuses it as the
- * diagonal sum of the rows above and below)
+ * the sum of a row's two horizontal neighbours of the centre column and
+ * G(r) = fma(H(r), 1/sqrt2, C(r)) its centre plus its diagonal share:
+ *   L = (G(up row) + G(down row)) + H(mid);  L = fma(-F, C, L);
+ *   C' = fma(k, L, C)
+ * (a column walking down the rows computes each H and G once; 7 operations
+ * per site instead of the 8 of (up + down) + H(mid) + (H(up) + H(down))/sqrt2)
  * with F = (float)(4 + (float)(2 sqrt2)) and k = ((D dt) HS) / F in float, the
  * coefficient expression of diffusion.cu:163. */
 static float eo_stencil_f32(const float *t, int i, int tw, float k) {
@@ -435,9 +436,9 @@ static float eo_stencil_f32(const float *t, int i, int tw, float k) {
   float hu = t[i - tw - 1] + t[i - tw + 1];
   float hm = t[i - 1] + t[i + 1];
   float hd = t[i + tw - 1] + t[i + tw + 1];
-  float o = (t[i - tw] + t[i + tw]) + hm;
-  float d = hu + hd;
-  float L = fmaf(d, R2, o);
+  float gu = fmaf(hu, R2, t[i - tw]);
+  float gd = fmaf(hd, R2, t[i + tw]);
+  float L = (gu + gd) + hm;
   L = fmaf(-F, t[i], L);
   return fmaf(k, L, t[i]);
 }
