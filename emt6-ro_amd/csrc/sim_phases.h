@@ -924,11 +924,11 @@ struct LaneRunDiffuser {
         const int nf = LR::layout((h + Lc - 1) / Lc, fin_runs, -1, dummy);
         if (n > slots[ci] * NT || nf > slots[ci] * NT) continue;
         // POLICY 0: fewest wave instructions; 1: most waves (lowest bands);
-        // 2: fewest wave instructions by the compiled stencil's count, 8 L +
-        // 5 VALU per substrate and lane slot (3 x 37 at L = 4 in the ISA).
+        // 2: fewest wave instructions by the compiled stencil's count, 7 L +
+        // 6 VALU per substrate and lane slot (3 x 34 + 4 at L = 4 in the ISA).
         const int waves = (n + 63) / 64;
         const int cost = POLICY == 0 ? waves * (24 * Lc + 30)
-                         : POLICY == 2 ? waves * (24 * Lc + 15) : Lc;
+                         : POLICY == 2 ? waves * (21 * Lc + 18) : Lc;
         if (best_cost < 0 || cost < best_cost) { best = Lc; best_cost = cost; }
       }
     }
