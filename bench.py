@@ -388,7 +388,10 @@ def main():
     }
     hours_sorted = sorted(hour_ms)
     out["persistent_hour_ms"] = {"first": hour_ms[hours_sorted[0]],
-                                 "last": hour_ms[hours_sorted[-1]]}
+                                 "last": hour_ms[hours_sorted[-1]],
+                                 "per_12h": [round(sum(hour_ms.get(h, 0.0)
+                                                       for h in range(h0, h0 + 12)), 1)
+                                             for h0 in range(0, HOURS, 12)]}
     if not args.no_plain:
         # The per-step path on every N-th hour of the same run (the other hours
         # advance on the persistent path, untimed); both paths give identical
