@@ -328,3 +328,34 @@ def test_empty_and_tiny_lattices_match_oracle(sim, params, path):
     assert got[0] == 0 and got[2] == 0 and got[1] > 0
     for i in range(n):
         compare_states(b.state(i), orc, i, f"{path} sim {i}")
+
+
+@pytest.mark.parametrize("path", ["persistent", "graph"])
+def test_colonies_dying_out_mid_run_match_oracle(sim, params, path):
+    """10 Gy at step 0 on a radius-3 colony (31 cells): the repair delay
+    (3.3414 exp(0.1492 R), about 15 h) ends in death with probability
+    1 - exp(-0.4993 R) = 0.993, so some replicates die out between hours 32
+    and 34 and the others keep a few cells.  The ROI of a dying colony shrinks
+    to nothing mid-run (diffusion then skipped); every state matches the
+    oracle bit for bit before (24 h) and after (48 h)."""
+    from emt6ro.batch import Batch
+    from emt6ro.synthetic import disk_state
+    v = params.values()
+    colony, arr = disk_state(params, 53, 53, radius=3, seed=9)
+    doses = np.zeros((1, 480), np.float32)
+    doses[0, 0] = 10.0
+    n, seed = 8, 5
+    b = Batch(params, [colony], np.zeros(n, np.int64), np.zeros(n, np.int64),
+              np.full(n, 2 ** 32 - 1, np.uint32), doses, 300, 144000, sim_id=np.arange(n),
+              seed=seed)
+    orc = O.OracleBatch(v, [oracle_state_from_arrays(arr, v)] * n, doses, np.zeros(n, np.int64),
+                        np.arange(n), 300, 144000)
+    for hours in (24, 24):
+        log(f"[parity] colonies dying out, path={path}, {hours} more hours")
+        b.run(hours * 600, path=path)
+        orc.run(hours * 600, seed=seed, nthreads=4)
+        got = b.results().astype(np.int64)
+        assert np.array_equal(got, orc.counts()), (got, orc.counts())
+        for i in range(n):
+            compare_states(b.state(i), orc, i, f"{path} sim {i}")
+    assert (got == 0).any() and (got > 0).any(), got
