@@ -359,3 +359,44 @@ def test_colonies_dying_out_mid_run_match_oracle(sim, params, path):
         for i in range(n):
             compare_states(b.state(i), orc, i, f"{path} sim {i}")
     assert (got == 0).any() and (got > 0).any(), got
+
+
+@pytest.mark.parametrize("path", ["persistent", "graph"])
+def test_colonies_at_the_lattice_edge_match_oracle(sim, params, path):
+    """Colonies placed in a corner, against an edge, at the bottom and in the
+    opposite corner of a 53x53 lattice, beside a centred one: their ROIs are
+    clipped at the first or last interior row or column, so the temporary
+    grid's rings (and, on the persistent path, the LDS window placed around
+    the ROI) reach past the lattice.  1.5 Gy at step 300; every state matches
+    the oracle bit for bit at 1 h and 12 h."""
+    from emt6ro.batch import Batch
+    from emt6ro.synthetic import disk_state
+    v = params.values()
+    spec = [((4, 4), 5), ((26, 2), 4), ((49, 30), 6), ((2, 50), 3), ((26, 26), 6)]
+    tumors, states = [], []
+    for k, (center, radius) in enumerate(spec):
+        st, arr = disk_state(params, 53, 53, radius=radius, seed=30 + k, center=center)
+        tumors.append(st)
+        states.append(oracle_state_from_arrays(arr, v))
+    doses = np.zeros((1, 480), np.float32)
+    doses[0, 1] = 1.5
+    reps, seed = 2, 11
+    tumor_index = np.repeat(np.arange(len(spec)), reps)
+    n = len(tumor_index)
+    b = Batch(params, tumors, tumor_index, np.zeros(n, np.int64),
+              np.full(n, 2 ** 32 - 1, np.uint32), doses, 300, 144000, sim_id=np.arange(n),
+              seed=seed)
+    orc = O.OracleBatch(v, [states[t] for t in tumor_index], doses, np.zeros(n, np.int64),
+                        np.arange(n), 300, 144000)
+    clipped = 0
+    for hours in (1, 11):
+        log(f"[parity] colonies at the lattice edge, path={path}, {hours} more hours")
+        b.run(hours * 600, path=path)
+        orc.run(hours * 600, seed=seed, nthreads=4)
+        got = b.results().astype(np.int64)
+        assert np.array_equal(got, orc.counts()), (got, orc.counts())
+        for i in range(n):
+            compare_states(b.state(i), orc, i, f"{path} sim {i}")
+        r0, c0, h, w = (orc.roi[:, k] for k in range(4))
+        clipped = max(clipped, int(((r0 == 1) | (c0 == 1) | (r0 + h == 52) | (c0 + w == 52)).sum()))
+    assert clipped >= 6, orc.roi.tolist()
