@@ -70,8 +70,8 @@ Batch::Batch(const HostParams &p, const std::vector<const TumorState *> &tumors,
                   spec.sim_id.size() == (size_t)nsims_, -1, "Inconsistent batch arrays");
   EMT_ENFORCE(spec.prot_res > 0, -1, "Protocol resolution must be positive");
   EMT_ENFORCE(spec.n_prot > 0 && spec.n_slots > 0, -1, "Protocol table must be non-empty");
-  EMT_HIP(hipSetDevice(device_));
-  // Per-tumour images.
+  // Per-tumour images (host only: every argument is checked before the
+  // first device call).
   const int T = (int)tumors.size();
   std::vector<ImgDesc> img(T);
   int64_t isub = 0, ist = 0, islot = 0;
@@ -163,6 +163,7 @@ Batch::Batch(const HostParams &p, const std::vector<const TumorState *> &tumors,
     c.hmax = std::max(c.hmax, d.H);
     c.wmax = std::max(c.wmax, d.W);
   }
+  EMT_HIP(hipSetDevice(device_));
   EMT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   EMT_HIP(hipEventCreate(&ev0_));
   EMT_HIP(hipEventCreate(&ev1_));

@@ -31,6 +31,10 @@ class Batch:
         d = np.ascontiguousarray(doses, np.float32)
         if d.ndim != 2:
             raise ValueError("doses must be (n_protocols, n_slots)")
+        if not (ti.ndim == pi.ndim == hz.ndim == sid.ndim == 1) or \
+                not (len(pi) == len(hz) == len(sid) == n):
+            raise ValueError("tumor_index, prot_index, horizon and sim_id must be 1-D arrays "
+                             "of one length")
         arr = (C.c_void_p * len(self._tumors))(*[t._h.value for t in self._tumors])
         h = C.c_void_p()
         check(lib.emt6ro_batch_create(params._h, arr, len(self._tumors), n,
