@@ -56,3 +56,28 @@ def test_batch_rejects_arrays_of_different_lengths(setup):
         _batch(params, [ok], [0, 0], [0, 0], hz=np.full(1, 10, np.uint32))
     with pytest.raises(ValueError, match="one length"):
         _batch(params, [ok], [0, 0], [0, 0], sid=np.arange(3))
+
+
+def test_experiment_rejects_bad_arguments(setup):
+    """Experiment (the reference's drop-in, built on a batch) rejects the
+    same way: no tumours or zero counts, lattices over 86 x 86, tumours of
+    different sizes in one experiment (the reference's Experiment sizes every
+    simulation by the parameters' lattice dimensions, backend.cc:37-52)."""
+    import emt6ro.simulation as sim
+    params, ok = setup
+    for tumors, reps, prots in (([], 1, 2), ([ok], 0, 2), ([ok], 1, 0)):
+        with pytest.raises(RuntimeError, match="counts must be positive"):
+            sim.Experiment(params, tumors, reps, prots, simulation_steps=600, seed=0)
+    with pytest.raises(RuntimeError, match="Lattices are limited to 86 x 86"):
+        sim.Experiment(params, [sim.TumorState.empty(params, 87, 87)], 1, 1,
+                       simulation_steps=600, seed=0)
+    with pytest.raises(RuntimeError, match="same dimensions"):
+        sim.Experiment(params, [ok, sim.TumorState.empty(params, 33, 33)], 1, 1,
+                       simulation_steps=600, seed=0)
+
+
+def test_surrogate_network_rejects_bad_shapes():
+    from emt6ro.surrogate import MLP
+    for shape in ((0, 512), (71, 0), (71, 100)):
+        with pytest.raises(RuntimeError, match="multiple of 64"):
+            MLP(*shape)
