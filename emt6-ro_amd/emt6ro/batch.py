@@ -31,6 +31,7 @@ class Batch:
         d = np.ascontiguousarray(doses, np.float32)
         if d.ndim != 2:
             raise ValueError("doses must be (n_protocols, n_slots)")
+        self._dose_shape = d.shape
         if not (ti.ndim == pi.ndim == hz.ndim == sid.ndim == 1) or \
                 not (len(pi) == len(hz) == len(sid) == n):
             raise ValueError("tumor_index, prot_index, horizon and sim_id must be 1-D arrays "
@@ -92,7 +93,11 @@ class Batch:
         self.window = int(rows)
 
     def set_doses(self, doses):
+        """Replace the protocol table; same shape as the one the batch was
+        made with."""
         d = np.ascontiguousarray(doses, np.float32)
+        if d.shape != self._dose_shape:
+            raise ValueError(f"doses must be {self._dose_shape}, got {d.shape}")
         check(lib.emt6ro_batch_set_doses(self._h, _p(d, C.c_float)))
 
     def __del__(self):

@@ -249,6 +249,8 @@ class MLP:
         """(loss, gradient) of loss = sum((f(X) - y)^2) / (2 n), no update."""
         x = self._x(X)
         t = np.ascontiguousarray(y, np.float32).reshape(-1)
+        if t.size != x.shape[0]:
+            raise ValueError("X and y differ in length")
         g = np.zeros(self.num_params, np.float32)
         loss = C.c_float()
         check_mlp(lib.emt6ro_mlp_grads(self._h, _fp(x), _fp(t), x.shape[0], _fp(g),

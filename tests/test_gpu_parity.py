@@ -299,6 +299,8 @@ def test_reset_restores_initial_state(sim, params):
     b.reset(epoch=0)
     b.run(700)
     assert np.array_equal(b.results(), first)
+    with pytest.raises(ValueError, match="doses must be"):
+        b.set_doses(dose_table()[:, :-1])
 
 
 @pytest.mark.parametrize("path", ["persistent", "step"])
@@ -400,3 +402,44 @@ def test_colonies_at_the_lattice_edge_match_oracle(sim, params, path):
         r0, c0, h, w = (orc.roi[:, k] for k in range(4))
         clipped = max(clipped, int(((r0 == 1) | (c0 == 1) | (r0 + h == 52) | (c0 + w == 52)).sum()))
     assert clipped >= 6, orc.roi.tolist()
+
+
+@pytest.mark.parametrize("path", ["persistent", "graph"])
+def test_protocol_length_and_resolution_match_oracle(sim, params, path):
+    """A dose fires at step s only when s is a multiple of the protocol
+    resolution and s < the protocol length (protocol.h:9-14).  Resolution
+    150, length 600: slots 0-3 fire and slot 4 (step 600) does not, nor does
+    slot 7 beyond it.  Both paths match the oracle bit for bit after 1200
+    steps; on the oracle, the doses in slots 4-7 change nothing, while a
+    length of 601 lets slot 4 fire and changes the cells' irradiation."""
+    from emt6ro.batch import Batch
+    ins = inputs(sim, params)
+    v = params.values()
+    doses = np.zeros((2, 8), np.float32)
+    doses[0, [0, 2, 4, 7]] = [1.0, 2.0, 3.0, 4.0]
+    doses[1, :] = 0.75
+    T = len(ins)
+    tumor_index = np.repeat(np.arange(T), 2)
+    prot_index = np.tile(np.arange(2), T)
+    n, steps, seed = len(tumor_index), 1200, 21
+    b = Batch(params, [s for s, _ in ins], tumor_index, prot_index,
+              np.full(n, 2 ** 32 - 1, np.uint32), doses, 150, 600, sim_id=np.arange(n), seed=seed)
+    log(f"[parity] protocol resolution 150, length 600, path={path}")
+    b.run(steps, path=path)
+
+    def oracle(d, length):
+        orc = O.OracleBatch(v, [ins[t][1] for t in tumor_index], d, prot_index, np.arange(n),
+                            150, length)
+        orc.run(steps, seed=seed, nthreads=4)
+        return orc
+
+    orc = oracle(doses, 600)
+    got = b.results().astype(np.int64)
+    assert np.array_equal(got, orc.counts()), (got, orc.counts())
+    for i in range(n):
+        compare_states(b.state(i), orc, i, f"{path} sim {i}")
+    cut = doses.copy()
+    cut[:, 4:] = 0.0
+    same = oracle(cut, 600)
+    assert np.array_equal(same.sub, orc.sub) and np.array_equal(same.cells["R"], orc.cells["R"])
+    assert not np.array_equal(oracle(doses, 601).cells["R"], orc.cells["R"])

@@ -60,6 +60,12 @@ def test_params_round_trip_and_count(S):
         net.set_params(q[:-1])
     with pytest.raises(RuntimeError, match="multiple of 64"):
         S.MLP(N_IN, 100)
+    X = np.zeros((8, N_IN), np.float32)
+    for call in (lambda: net.grads(X, np.zeros(7, np.float32)),
+                 lambda: net.set_data(X, np.zeros(9, np.float32)),
+                 lambda: net.predict(np.zeros((8, N_IN + 1), np.float32))):
+        with pytest.raises(ValueError):
+            call()
 
 
 @pytest.mark.parametrize("n", [1, 300, 4096 + 17])
