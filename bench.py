@@ -68,8 +68,10 @@ def parse():
     ap.add_argument("--sims", type=int, default=0,
                     help="use only the first N simulations (profiling runs)")
     ap.add_argument("--tune-steps", type=int, default=600)
-    ap.add_argument("--tune-refine-warm", type=int, default=1800,
-                    help="re-time the closest variants after this many steps (0: off)")
+    ap.add_argument("--tune-refine-warm", type=int, default=HOURS * HOUR // 2,
+                    help="re-time the closest variants after this many steps (0: off); "
+                         "default mid-run (72 h), where the cost per hour is near the "
+                         "run's mean")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the surrogate training / screening sample")
     ap.add_argument("--surrogate-only", action="store_true",

@@ -352,7 +352,7 @@ int Batch::choose_variant(int hmax, int wmax, int kind) {
   // Default without tuning: the variants that tuned fastest on MI355X for
   // lattices of each row stride (36: 33x33, 56: 53x53, 88: 81x81), in
   // order of preference, at the smallest row stride that fits.
-  static const int kPreferred[] = {36, 35, 39, 32, 43, 42, 41, 34, 25, 24, 14, 17, 19, 11};
+  static const int kPreferred[] = {49, 36, 35, 39, 32, 43, 42, 41, 34, 25, 24, 14, 17, 19, 11};
   for (int id : kPreferred) {
     if (id >= cnt || v[id].rs != v[best].rs || !variant_fits(v[id], hmax, wmax)) continue;
     if (persistent_occupancy(id, persistent_lds_bytes(v[id], hmax, wmax)) == 0) continue;

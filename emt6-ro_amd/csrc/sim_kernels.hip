@@ -363,6 +363,7 @@ template <> struct LaneRunShape<5> { static constexpr int h1 = 3, h2 = 4, h3 = 5
 template <> struct LaneRunShape<6> { static constexpr int h1 = 4, h2 = 6, h3 = 8, pol = 1, unroll = 2; };
 template <> struct LaneRunShape<7> { static constexpr int h1 = 3, h2 = 4, h3 = 5, pol = 2, unroll = 2; };
 template <> struct LaneRunShape<8> { static constexpr int h1 = 3, h2 = 4, h3 = 6, pol = 1, unroll = 2; };
+template <> struct LaneRunShape<9> { static constexpr int h1 = 4, h2 = 6, h3 = 8, pol = 3, unroll = 1; };
 template <int NT, int A, int B, int RS, int PS>
 struct DiffSel<3, NT, A, B, RS, PS> {
   using S = LaneRunShape<B>;
@@ -648,7 +649,7 @@ hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds
 // workgroup size, A, B (kind 0: band rows, items per thread; kind 1: Jacobi
 // and final sites per thread; kind 3: A = lane slots per thread, B = band
 // heights, policy and substep unroll (LaneRunShape; 45-48 are 32, 44, 43
-// and 39 with the substeps in pairs); kind 4:
+// and 39 with the substeps in pairs, 49 is 32 with band policy 3); kind 4:
 // A = lane slots per thread, B = heights 4/6/8 (0) or 2/4/8 preferring low
 // bands (1)), ring-save
 // slots per thread, LDS row stride (lattices up to stride-2 on a side),
@@ -703,7 +704,8 @@ hipError_t launch_small(bool cells, const dev::RunArgs &a, int nsims, size_t lds
   X(45, 3, 256, 2, 6, 2, 56, 4)   \
   X(46, 3, 256, 3, 7, 2, 56, 4)   \
   X(47, 3, 128, 2, 6, 3, 36, 4)   \
-  X(48, 3, 512, 3, 8, 2, 88, 2)
+  X(48, 3, 512, 3, 8, 2, 88, 2)   \
+  X(49, 3, 256, 2, 9, 2, 56, 4)
 
 const VariantInfo *variant_table(int *count) {
   static const VariantInfo table[] = {
@@ -819,14 +821,16 @@ hipError_t launch_order_by_cost(const int32_t *src, int32_t *dst, int n, const S
 
 // The window kernels, lane-run diffusion, 4 waves per SIMD: row stride 36
 // with 128 threads (eight workgroups per CU), band heights 3/4/5 chosen by
-// the stencil's instruction count, 3 lane slots (1.3% faster over the whole
-// 53x53 workload than heights 4/6/8, lowest first, in 2 slots); row stride 56
+// the stencil's instruction count with the busiest wave's share weighted
+// (policy 3), 3 lane slots (over the whole 53x53 workload, heights 3/4/5 by
+// instruction count alone were 1.3% faster than heights 4/6/8, lowest first,
+// in 2 slots, and policy 3 0.8% faster than that); row stride 56
 // with 256 threads (four per CU, for lattices wider than 54 sites, whose
 // whole-lattice kernel holds one workgroup per CU), heights 4/6/8.  (A
 // 44-column window at six 128-thread workgroups per CU ran slower than the
 // 53x53 whole-lattice kernel.)
 #ifndef EMT6RO_WINDOW36_SHAPE
-#define EMT6RO_WINDOW36_SHAPE 3, 3, 4, 5, 2
+#define EMT6RO_WINDOW36_SHAPE 3, 3, 4, 5, 3
 #endif
 using WindowDiff36 = dev::LaneRunDiffuser<128, 36, 3, EMT6RO_WINDOW36_SHAPE>;
 using WindowDiff56 = dev::LaneRunDiffuser<256, 56, 2, 2, 4, 6, 8, 1>;

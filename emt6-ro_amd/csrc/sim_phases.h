@@ -925,10 +925,14 @@ struct LaneRunDiffuser {
         if (n > slots[ci] * NT || nf > slots[ci] * NT) continue;
         // POLICY 0: fewest wave instructions; 1: most waves (lowest bands);
         // 2: fewest wave instructions by the compiled stencil's count, 7 L +
-        // 6 VALU per substrate and lane slot (3 x 34 + 4 at L = 4 in the ISA).
+        // 6 VALU per substrate and lane slot (3 x 34 + 4 at L = 4 in the ISA);
+        // 3: as 2 plus four times the slots of the busiest wave (the
+        // workgroup's first: lane slots fill thread by thread), which bounds
+        // the time between the substep's barriers.
         const int waves = (n + 63) / 64;
         const int cost = POLICY == 0 ? waves * (24 * Lc + 30)
-                         : POLICY == 2 ? waves * (21 * Lc + 18) : Lc;
+                         : POLICY == 2 ? waves * (21 * Lc + 18)
+                         : POLICY == 3 ? (4 * ((n + NT - 1) / NT) + waves) * (21 * Lc + 18) : Lc;
         if (best_cost < 0 || cost < best_cost) { best = Lc; best_cost = cost; }
       }
     }

@@ -44,12 +44,15 @@ def autotune(batch, classes, steps=150, use_cache=True, log=None, refine_warm=0,
     after refine_warm steps run with the fastest one, since tumours grow and
     the ranking of close variants can change with the ROI size; results do
     not depend on the variant, so every candidate starts from the same
-    state."""
+    state.  The refinement runs with the caller's window setting, so that
+    the whole-lattice kernel sees the simulations it takes in production
+    (those whose ROI no longer fits a window)."""
     cache = _load() if use_cache else {}
-    # The variants are timed on the whole-lattice kernel alone: the window
-    # pass (which takes the simulations with small ROIs first, so that the
-    # whole-lattice kernel would see none of them early in a run) stays off
-    # until the end, when the caller's setting is restored.
+    # The first timing is of the whole-lattice kernel alone: the window pass
+    # (which takes the simulations with small ROIs first, so that the
+    # whole-lattice kernel would see none of them early in a run) is off
+    # except during a refinement, and the caller's setting is restored at
+    # the end.
     window = getattr(batch, "window", -1)
     if hasattr(batch, "set_window"):
         batch.set_window(0)
@@ -78,10 +81,14 @@ def autotune(batch, classes, steps=150, use_cache=True, log=None, refine_warm=0,
             later = {}
             for _, v in close:
                 batch.reset(0)
+                if hasattr(batch, "set_window"):
+                    batch.set_window(window)
                 batch.set_variant(rs, best)
                 batch.run(refine_warm, path="persistent")
                 batch.set_variant(rs, v)
                 later[v] = batch.run(steps, path="persistent")
+                if hasattr(batch, "set_window"):
+                    batch.set_window(0)
                 if log:
                     log(f"[tune] rs={rs} variant {v} after {refine_warm} steps: {later[v]:.2f} ms")
             best = min(later, key=later.get)
