@@ -59,7 +59,9 @@ def batched_uniform(params, n_tumors=10, n_protocols=100, reps=20, hours=144, se
     tumors = synthetic_tumors(params, n_tumors, seed)
     prots = random_protocols(n_protocols, seed + 1)
     horizon = hours * STEPS_PER_HOUR
-    n_slots = (horizon + SLOT_STEPS - 1) // SLOT_STEPS
+    # every protocol's slots (doses past the horizon never fire)
+    last = max(s for p in prots for s, _ in p)
+    n_slots = max((horizon + SLOT_STEPS - 1) // SLOT_STEPS, last // SLOT_STEPS + 1)
     P, T, R = n_protocols, n_tumors, reps
     b = np.arange(P * T * R)
     return {
