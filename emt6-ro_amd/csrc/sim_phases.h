@@ -929,10 +929,10 @@ struct LaneRunDiffuser {
         // 3: as 2 plus four times the slots of the busiest wave (the
         // workgroup's first: lane slots fill thread by thread), which bounds
         // the time between the substep's barriers.
-        const int waves = (n + 63) / 64;
+        const int waves = (n + 63) / 64, busiest = (n + NT - 1) / NT;
         const int cost = POLICY == 0 ? waves * (24 * Lc + 30)
                          : POLICY == 2 ? waves * (21 * Lc + 18)
-                         : POLICY == 3 ? (4 * ((n + NT - 1) / NT) + waves) * (21 * Lc + 18) : Lc;
+                         : POLICY == 3 ? (4 * busiest + waves) * (21 * Lc + 18) : Lc;
         if (best_cost < 0 || cost < best_cost) { best = Lc; best_cost = cost; }
       }
     }
